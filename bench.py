@@ -1,0 +1,315 @@
+"""Benchmark of the fused-expression back-end on one MI355X.
+
+Headline (the `value` of the JSON line): the view-heavy fused chain
+    X [64,256,256] ~ N(0,1), bias [256] ~ U(-0.5,0.5)
+    p = X.permute(1,0,2); y = p + bias.broadcast([256,64,256]) + p.translate([0,0,1])
+    out = exp(-(y*y)) * tanh(y)
+as one generated kernel, in GB/s of algorithmic traffic (X read once, bias, out written:
+33,555,456 bytes per launch). One step = one launch over one input set; 16 input/output
+sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch streams
+from HBM. The K timed launches are replayed from a captured hipGraph and bracketed by HIP
+events on the launch stream.
+
+The same line carries the other workloads under "also": a*b+c on [1024,1024] and its sum,
+the one-pass row sum/row max/total over [16384,4096], and C = A.B at 4096^3 per GEMM
+variant with its FP64 error. `cpu_baseline` times the reference's generated kernel for the
+headline workload (its OpenCL C text run as host C on OpenMP threads) on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c1,c3,c4]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "compute.scala_amd"))
+sys.path.insert(0, ROOT)
+
+METRIC = "fused-expr HBM GB/s (% of 8 TB/s); GEMM TFLOP/s at err≤1e-5 vs FP64; vs plain"
+HBM_PEAK_GBS = 8000.0
+PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0}
+
+
+def log(msg):
+    sys.stderr.write("[bench] %s\n" % msg)
+    sys.stderr.flush()
+
+
+class Timer(object):
+    """HIP events on the launch stream."""
+
+    def __init__(self, capi, stream):
+        self.capi, self.stream = capi, stream
+
+    def measure(self, fn, reps):
+        lib = self.capi.lib
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        self.capi.check(lib.cs_stream_synchronize(self.stream))
+        self.capi.check(lib.cs_event_record(self.stream, ctypes.byref(e0)))
+        for _ in range(reps):
+            fn()
+        self.capi.check(lib.cs_event_record(self.stream, ctypes.byref(e1)))
+        self.capi.check(lib.cs_event_wait(e1))
+        self.capi.check(lib.cs_stream_synchronize(self.stream))
+        ms = ctypes.c_float()
+        self.capi.check(lib.cs_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        lib.cs_event_release(e0)
+        lib.cs_event_release(e1)
+        return ms.value
+
+
+def capture(capi, stream, fns):
+    lib = capi.lib
+    capi.check(lib.cs_graph_begin_capture(stream))
+    for f in fns:
+        f()
+    g = ctypes.c_void_p()
+    capi.check(lib.cs_graph_end_capture(stream, ctypes.byref(g)))
+    return g
+
+
+def bench_c2(T, capi, steps, warmup, sets=16):
+    from compute import tensors as tz
+    lib = capi.lib
+    n = 64 * 256 * 256
+    xs = [T._alloc(n * 4) for _ in range(sets)]
+    bs = [T._alloc(256 * 4) for _ in range(sets)]
+    outs = [T._alloc(n * 4) for _ in range(sets)]
+    for i in range(sets):
+        capi.check(lib.cs_fill_normal(T.stream, xs[i].ptr, n, 2001 + 97 * i))
+        capi.check(lib.cs_fill_uniform(T.stream, bs[i].ptr, 256, 2002 + 97 * i, -0.5, 0.5))
+    # The kernel is the one the Tensor API generates for the expression.
+    X = tz.CachedTensor(T, (64, 256, 256), 0.0, xs[0].retain())
+    B = tz.CachedTensor(T, (256,), 0.0, bs[0].retain())
+    p = X.permute([1, 0, 2])
+    y = p + B.broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+    out = T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+    f, leaves, _ = T.compile(out.closure, out.shape)
+    assert [l.id for l in leaves] == [X, B]
+    ptrs = [(ctypes.c_void_p * 2)(xs[i].ptr, bs[i].ptr) for i in range(sets)]
+
+    def launch(i):
+        capi.check(lib.cs_fused_launch(T.stream, f, ptrs[i], 2, outs[i].ptr))
+
+    for i in range(sets):
+        launch(i)
+    T.synchronize()
+    g = capture(capi, T.stream, [lambda i=i: launch(i) for i in range(sets)])
+    timer = Timer(capi, T.stream)
+    reps_w = max(1, warmup // sets)
+    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps_w)
+    reps = max(1, steps // sets)
+    ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
+    launches = reps * sets
+    # hot-cache figure: one input set, repeated
+    g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
+    ms_hot = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_hot)), reps)
+    bytes_per = 4 * n + 4 * 256 + 4 * n
+    res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
+               gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
+               gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9)
+    lib.cs_graph_release(g)
+    lib.cs_graph_release(g_hot)
+    lib.cs_fused_release(f)
+    X.release()
+    B.release()
+    for h in xs + bs + outs:
+        h.release()
+    return res
+
+
+def bench_c1(T, capi, steps, sets=16):
+    lib = capi.lib
+    n = 1024 * 1024
+    a = [[T._alloc(n * 4) for _ in range(3)] for _ in range(sets)]
+    outs = [T._alloc(n * 4) for _ in range(sets)]
+    sums = [T._alloc(4) for _ in range(sets)]
+    for i in range(sets):
+        for j in range(3):
+            capi.check(lib.cs_fill_uniform(T.stream, a[i][j].ptr, n, 1001 + j + 97 * i, -1.0, 1.0))
+    from compute import tensors as tz
+    ta, tb, tc = (tz.CachedTensor(T, (1024, 1024), 0.0, a[0][j].retain()) for j in range(3))
+    e = ta * tb + tc
+    f, leaves, _ = T.compile(e.closure, e.shape)
+    ptrs = [(ctypes.c_void_p * 3)(*[a[i][j].ptr for j in range(3)]) for i in range(sets)]
+    for i in range(sets):
+        capi.check(lib.cs_fused_launch(T.stream, f, ptrs[i], 3, outs[i].ptr))
+        capi.check(lib.cs_reduce_sum(T.stream, outs[i].ptr, n, sums[i].ptr, None))
+    T.synchronize()
+    timer = Timer(capi, T.stream)
+    reps = max(1, steps // sets)
+    g1 = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_fused_launch(T.stream, f, ptrs[i], 3, outs[i].ptr))
+                                  for i in range(sets)])
+    timer.measure(lambda: lib.cs_graph_launch(T.stream, g1), 2)
+    ms_fma = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g1)), reps)
+    g2 = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_reduce_sum(T.stream, outs[i].ptr, n, sums[i].ptr, None))
+                                  for i in range(sets)])
+    timer.measure(lambda: lib.cs_graph_launch(T.stream, g2), 2)
+    ms_sum = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g2)), reps)
+    L = reps * sets
+    res = {"fma_gbs": 16 * n * L / (ms_fma * 1e-3) / 1e9, "fma_us": ms_fma * 1e3 / L,
+           "sum_gbs": (4 * n + 4) * L / (ms_sum * 1e-3) / 1e9, "sum_us": ms_sum * 1e3 / L}
+    for g in (g1, g2):
+        lib.cs_graph_release(g)
+    lib.cs_fused_release(f)
+    for t in (ta, tb, tc):
+        t.release()
+    for h in [x for s in a for x in s] + outs + sums:
+        h.release()
+    return res
+
+
+def bench_c3(T, capi, steps, sets=2):
+    lib = capi.lib
+    rows, cols = 16384, 4096
+    n = rows * cols
+    ins = [T._alloc(n * 4) for _ in range(sets)]
+    rs = [T._alloc(rows * 4) for _ in range(sets)]
+    rm = [T._alloc(rows * 4) for _ in range(sets)]
+    tot = [T._alloc(8) for _ in range(sets)]
+    for i in range(sets):
+        capi.check(lib.cs_fill_normal(T.stream, ins[i].ptr, n, 3001 + 97 * i))
+
+    def launch(i):
+        capi.check(lib.cs_reduce_rows(T.stream, ins[i].ptr, rows, cols, rs[i].ptr, rm[i].ptr, None, tot[i].ptr))
+
+    for i in range(sets):
+        launch(i)
+    T.synchronize()
+    g = capture(capi, T.stream, [lambda i=i: launch(i) for i in range(sets)] * 4)
+    timer = Timer(capi, T.stream)
+    timer.measure(lambda: lib.cs_graph_launch(T.stream, g), 1)
+    reps = max(1, steps // (sets * 4))
+    ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
+    L = reps * sets * 4
+    bytes_per = 4 * n + 4 * rows * 2 + 8
+    res = {"rows_gbs": bytes_per * L / (ms * 1e-3) / 1e9, "rows_us": ms * 1e3 / L, "bytes_per_launch": bytes_per}
+    lib.cs_graph_release(g)
+    for h in ins + rs + rm + tot:
+        h.release()
+    return res
+
+
+def bench_gemm(T, capi, variants, reps=10):
+    lib = capi.lib
+    n = 4096
+    from oracle import fp64, inputs
+    a, b = inputs.c4_inputs(n)
+    ba, bb, bc = T._alloc(n * n * 4), T._alloc(n * n * 4), T._alloc(n * n * 4)
+    for buf, arr in ((ba, a), (bb, b)):
+        capi.check(lib.cs_buffer_write(T.stream, buf.handle, 0, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1))
+    rows = np.arange(0, n, 128)
+    c64 = a[rows].astype(np.float64) @ b.astype(np.float64)
+    timer = Timer(capi, T.stream)
+    out = {}
+    for name, v in variants:
+        def run():
+            capi.check(lib.cs_gemm(T.stream, v, n, n, n, ba.ptr, n, bb.ptr, n, bc.ptr, n))
+        run()
+        T.synchronize()
+        c = np.empty((n, n), np.float32)
+        capi.check(lib.cs_memcpy_dtoh(T.stream, c.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(bc.ptr), c.nbytes))
+        err = fp64.gemm_normalized_error(c[rows], a[rows], b, c64)
+        timer.measure(run, 2)
+        ms = timer.measure(run, reps)
+        tflops = 2.0 * n ** 3 * reps / (ms * 1e-3) / 1e12
+        out[name] = {"tflops": tflops, "ms": ms / reps, "normalized_error": err,
+                     "mfma_util": tflops * (3 if name == "bf16x3" else 1) / PEAK_TFLOPS[name]}
+        log("gemm %s: %.1f TFLOP/s, error %.3g" % (name, tflops, err))
+    for h in (ba, bb, bc):
+        h.release()
+    return out
+
+
+def cpu_baseline_c2(seconds=10.0):
+    """The reference's kernel for the headline expression (refcl-host, precise mode) on all
+    host cores, timed on whole launches until about `seconds` have passed."""
+    from compute import Tensors
+    from oracle import inputs, refcl
+    T = Tensors(offline=True)
+    X, bias = inputs.c2_inputs()
+    p = T.Tensor(X).permute([1, 0, 2])
+    y = p + T.Tensor(bias).broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+    out = T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+    k = refcl.Kernel(out.closure, out.shape, "precise")
+    k([X, bias])
+    runs, t0 = 0, time.perf_counter()
+    while True:
+        k([X, bias])
+        runs += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or runs >= 200:
+            break
+    bytes_per = 4 * X.size + 4 * 256 + 4 * X.size
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": round(bytes_per * runs / el / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "sample": "%d full launches of the workload's reference kernel text (OpenMP NDRange, host C, "
+                      "no FMA) in %.1f s" % (runs, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2048)
+    ap.add_argument("--warmup", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="c2,c1,c3,c4")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every workload here runs on a single MI355X, "
+              "so --gpus %d is not supported" % args.gpus)
+        sys.exit(2)
+    from compute import Tensors, _capi as capi
+    only = set(args.only.split(","))
+    T = Tensors(0)
+    log("device: %d CUs" % T.info(capi.INFO_COMPUTE_UNITS))
+    c2 = bench_c2(T, capi, args.steps, args.warmup)
+    log("c2: %.1f GB/s (%.2f us/launch), hot cache %.1f GB/s" % (c2["gbs"], c2["ms_per_launch"] * 1e3,
+                                                                c2["gbs_hot_cache"]))
+    also = {}
+    if "c1" in only:
+        also["c1_fma_sum"] = bench_c1(T, capi, args.steps)
+        log("c1: %s" % also["c1_fma_sum"])
+    if "c3" in only:
+        also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16))
+        log("c3: %s" % also["c3_rows_sum_max_total"])
+    if "c4" in only:
+        also["c4_gemm_4096"] = bench_gemm(T, capi, [("fp32", capi.GEMM_FP32)])
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline_c2()
+        log("cpu baseline: %s" % cpu)
+    gbs = c2["gbs"]
+    line = {
+        "metric": METRIC,
+        "value": round(gbs, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": c2["launches"],
+        "warmup": args.warmup,
+        "ms_per_step": c2["ms_per_launch"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Wang-hash seeded)",
+        "config": {"workload": "view-heavy fused chain: X[64,256,256] N(0,1), bias[256] U(-0.5,0.5); "
+                               "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
+                               "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
+                   "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2)},
+        "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "also": also,
+    }
+    T.close()
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,878 @@
+// Fused elementwise + view kernels: canonical DAG key, HIP code generation, the structural
+// kernel cache and launch.
+//
+// This is the MI355X code generator that takes the place of OpenCLKernelBuilder
+// (OpenCLKernelBuilder.scala:135-571) and of enqueueClosure's compile/cache/launch steps
+// (Tensors.scala:1263-1392). It keeps the reference's method: one kernel per expression
+// DAG, one SSA statement per operation, every view folded into the index arithmetic of
+// its leaf, padding outside the source bounds. What changes is the shape of the kernel:
+// a flattened 1-D grid where each thread produces `V` consecutive innermost outputs
+// (`U` times), contiguous leaves are read with one wide load, broadcast leaves with one
+// scalar load, and the outputs are written with wide coalesced stores.
+#include <cinttypes>
+#include <cmath>
+#include <cstdio>
+#include <sstream>
+
+#include "internal.h"
+
+struct cs_fused {
+  std::atomic<int> refs{1};
+  cs_device *device = nullptr;
+  std::string key;
+  uint64_t hash = 0;
+  std::string source;
+  hipModule_t module = nullptr;
+  hipFunction_t function = nullptr;
+  int nleaves = 0;
+  int64_t out_elements = 0;  // coordinates x tuple width
+  int64_t vectors = 0;       // number of V-wide output vectors
+  uint32_t block = 256;
+  uint32_t grid = 0;
+};
+
+namespace csmi {
+namespace {
+
+uint64_t fnv1a(const std::string &s) {
+  uint64_t h = 1469598103934665603ull;
+  for (unsigned char c : s) {
+    h ^= c;
+    h *= 1099511628211ull;
+  }
+  return h;
+}
+
+template <class T>
+void put(std::string &k, const T &v) {
+  k.append(reinterpret_cast<const char *>(&v), sizeof(T));
+}
+
+bool is_unary(int op) { return op >= CS_OP_NEG && op <= CS_OP_LOG; }
+bool is_binary(int op) { return op >= CS_OP_ADD && op <= CS_OP_MAX; }
+bool is_array(int op) { return op == CS_OP_PARAMETER || op == CS_OP_TRANSFORM || op == CS_OP_FILL; }
+
+// A validated, canonically ordered DAG.
+struct Dag {
+  std::vector<cs_node> nodes;     // canonical order: operands before users
+  std::vector<int> outputs;       // canonical ids
+  std::vector<int64_t> out_shape;
+  int nleaves = 0;
+  std::vector<std::vector<int64_t>> leaf_shape;  // per leaf slot
+};
+
+// The array a leaf gather reads: the parameter plus the composed view matrix, or none
+// for a bare parameter extract.
+struct ArrayRef {
+  int param = -1;            // canonical node id of the CS_OP_PARAMETER
+  bool is_fill = false;      // array of a repeated float
+  int fill_value = -1;       // float node id when is_fill
+  bool has_view = false;
+  int out_rank = 0;          // columns - 1 of matrix
+  std::vector<double> matrix;  // rank_src x (out_rank + 1)
+};
+
+// NDimensionalAffineTransform.concatenate (NDimensionalAffineTransform.scala:37-95):
+// m12 maps space 1 to space 2 (length2 rows, length1+1 columns), m01 maps space 0 to 1.
+std::vector<double> compose(const std::vector<double> &m12, const std::vector<double> &m01,
+                            int length2) {
+  int length1 = static_cast<int>(m12.size()) / length2 - 1;
+  int length0 = static_cast<int>(m01.size()) / length1 - 1;
+  std::vector<double> m02(static_cast<size_t>((length0 + 1) * length2), 0.0);
+  for (int i2 = 0; i2 < length2; ++i2) {
+    for (int i0 = 0; i0 < length0; ++i0) {
+      double acc = 0.0;
+      for (int i1 = 0; i1 < length1; ++i1)
+        acc += m12[i2 * (length1 + 1) + i1] * m01[i1 * (length0 + 1) + i0];
+      m02[i2 * (length0 + 1) + i0] = acc;
+    }
+    double acc = m12[i2 * (length1 + 1) + length1];
+    for (int i1 = 0; i1 < length1; ++i1)
+      acc += m12[i2 * (length1 + 1) + i1] * m01[i1 * (length0 + 1) + length0];
+    m02[i2 * (length0 + 1) + length0] = acc;
+  }
+  return m02;
+}
+
+// Validates the caller's nodes and renumbers the part reachable from the outputs in
+// first-visit depth-first order (operand a before b). Leaf slots must already follow that
+// order: it is the argument order the caller binds (Tensors.scala:230-251).
+Dag canonicalize(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutputs,
+                 int out_rank, const int32_t *out_shape) {
+  if (nnodes <= 0 || !nodes) fail(CS_INVALID_VALUE, "empty expression");
+  if (noutputs <= 0 || !outputs) fail(CS_INVALID_VALUE, "no outputs");
+  if (out_rank < 0 || out_rank > CS_MAX_RANK) fail(CS_INVALID_VALUE, "output rank out of range");
+  if (out_rank > 0 && !out_shape) fail(CS_INVALID_VALUE, "out_shape is null");
+  Dag dag;
+  int64_t total = 1;
+  for (int i = 0; i < out_rank; ++i) {
+    if (out_shape[i] < 0) fail(CS_INVALID_VALUE, "negative output dimension");
+    dag.out_shape.push_back(out_shape[i]);
+    total *= out_shape[i];
+  }
+  // Index arithmetic is 32-bit, as in the reference (int indices); outputs stay below 2^31.
+  if (total * noutputs >= (int64_t(1) << 31))
+    fail(CS_INVALID_BUFFER_SIZE, "fused outputs are limited to 2^31 - 1 elements");
+
+  std::vector<int> canon(static_cast<size_t>(nnodes), -1);
+  int next_leaf = 0;
+  // Iterative DFS: pre-order for leaf numbering, post-order for emission.
+  struct Frame { int node; int stage; };
+  for (int oi = 0; oi < noutputs; ++oi) {
+    int root = outputs[oi];
+    if (root < 0 || root >= nnodes) fail(CS_INVALID_VALUE, "output index out of range");
+    std::vector<Frame> stack{{root, 0}};
+    std::vector<char> on_stack(static_cast<size_t>(nnodes), 0);
+    while (!stack.empty()) {
+      Frame &f = stack.back();
+      int id = f.node;
+      if (canon[id] >= 0) { stack.pop_back(); continue; }
+      const cs_node &n = nodes[id];
+      if (f.stage == 0) {
+        if (on_stack[id]) fail(CS_INVALID_VALUE, "expression graph has a cycle");
+        on_stack[id] = 1;
+        if (n.op == CS_OP_PARAMETER) {
+          if (n.param != next_leaf)
+            fail(CS_INVALID_VALUE, "leaf slots must be numbered in first-visit order: expected " +
+                                       std::to_string(next_leaf) + ", got " + std::to_string(n.param));
+          if (n.rank < 0 || n.rank > CS_MAX_RANK) fail(CS_INVALID_VALUE, "leaf rank out of range");
+          next_leaf++;
+        }
+        f.stage = 1;
+        auto want = [&](int child) {
+          if (child < 0 || child >= nnodes) fail(CS_INVALID_VALUE, "operand index out of range");
+          if (canon[child] < 0) stack.push_back({child, 0});
+        };
+        // Push b first so that a is visited first.
+        bool two = is_binary(n.op);
+        bool one = is_unary(n.op) || n.op == CS_OP_TRANSFORM || n.op == CS_OP_FILL || n.op == CS_OP_EXTRACT;
+        if (!two && !one && n.op != CS_OP_LITERAL && n.op != CS_OP_PARAMETER)
+          fail(CS_INVALID_VALUE, "unknown op " + std::to_string(n.op));
+        if (two) { want(n.b); }
+        if (two || one) { want(n.a); }
+        // The vector may have grown: `f` is no longer valid here.
+        continue;
+      }
+      // post-order
+      cs_node c = n;
+      auto map = [&](int child) { return canon[child]; };
+      if (is_binary(n.op)) { c.a = map(n.a); c.b = map(n.b); }
+      else if (is_unary(n.op) || n.op == CS_OP_TRANSFORM || n.op == CS_OP_FILL || n.op == CS_OP_EXTRACT) { c.a = map(n.a); c.b = -1; }
+      else { c.a = c.b = -1; }
+      canon[id] = static_cast<int>(dag.nodes.size());
+      dag.nodes.push_back(c);
+      on_stack[id] = 0;
+      stack.pop_back();
+    }
+    dag.outputs.push_back(canon[root]);
+  }
+  dag.nleaves = next_leaf;
+  dag.leaf_shape.resize(static_cast<size_t>(next_leaf));
+  // Type checks.
+  for (auto &n : dag.nodes) {
+    auto arr = [&](int i) { return is_array(dag.nodes[i].op); };
+    switch (n.op) {
+      case CS_OP_PARAMETER:
+        dag.leaf_shape[n.param].assign(n.shape, n.shape + n.rank);
+        for (int i = 0; i < n.rank; ++i)
+          if (n.shape[i] < 0) fail(CS_INVALID_VALUE, "negative leaf dimension");
+        break;
+      case CS_OP_TRANSFORM:
+        if (!arr(n.a)) fail(CS_INVALID_VALUE, "transform of a non-array");
+        if (n.rank < 0 || n.rank > CS_MAX_RANK) fail(CS_INVALID_VALUE, "transform rank out of range");
+        break;
+      case CS_OP_FILL:
+        if (arr(n.a)) fail(CS_INVALID_VALUE, "fill of an array");
+        break;
+      case CS_OP_EXTRACT:
+        if (!arr(n.a)) fail(CS_INVALID_VALUE, "extract of a non-array");
+        break;
+      case CS_OP_LITERAL:
+        break;
+      default:
+        if (arr(n.a) || (is_binary(n.op) && arr(n.b)))
+          fail(CS_INVALID_VALUE, "arithmetic on an array");
+    }
+  }
+  for (int o : dag.outputs)
+    if (is_array(dag.nodes[o].op)) fail(CS_INVALID_VALUE, "an output must be a float term");
+  return dag;
+}
+
+// Rank of the source array a TRANSFORM chain ends in; views of fills stay fills.
+ArrayRef resolve_array(const Dag &dag, int id) {
+  const cs_node &n = dag.nodes[id];
+  if (n.op == CS_OP_PARAMETER) {
+    ArrayRef r;
+    r.param = id;
+    return r;
+  }
+  if (n.op == CS_OP_FILL) {
+    ArrayRef r;
+    r.is_fill = true;
+    r.fill_value = n.a;
+    return r;
+  }
+  // TRANSFORM: ArrayParameter.transform creates a view (OpenCLKernelBuilder.scala:431-433);
+  // a view's transform composes (OpenCLKernelBuilder.scala:334-340); a fill ignores it
+  // (OpenCLKernelBuilder.scala:491-493).
+  ArrayRef inner = resolve_array(dag, n.a);
+  if (inner.is_fill) return inner;
+  int src_rank = dag.nodes[inner.param].rank;
+  int cols = n.rank + 1;
+  int rows = inner.has_view ? inner.out_rank : src_rank;
+  std::vector<double> m(n.matrix, n.matrix + static_cast<size_t>(rows) * cols);
+  if (inner.has_view) {
+    m = compose(inner.matrix, m, src_rank);
+  }
+  inner.has_view = true;
+  inner.out_rank = n.rank;
+  inner.matrix = m;
+  return inner;
+}
+
+// java.text.DecimalFormat() with grouping off (OpenCLKernelBuilder.scala:14-32): at most
+// three fraction digits, HALF_EVEN on the exact binary value, no trailing zeros.
+std::string decimal_format(double v) {
+  char buf[512];
+  std::snprintf(buf, sizeof buf, "%.3f", v);  // glibc rounds the exact value, ties to even
+  std::string s = buf;
+  if (s.find('.') != std::string::npos) {
+    while (!s.empty() && s.back() == '0') s.pop_back();
+    if (!s.empty() && s.back() == '.') s.pop_back();
+  }
+  if (s == "-0") s = "-0";
+  return s;
+}
+
+std::string float_literal(float v) {
+  if (std::isnan(v)) return "__builtin_nanf(\"\")";
+  if (std::isinf(v)) return v > 0 ? "__builtin_huge_valf()" : "(-__builtin_huge_valf())";
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "%af", static_cast<double>(v));
+  return buf;
+}
+
+struct Options {
+  int math_mode = CS_MATH_PRECISE;
+  int V = 0, U = 0, block = 0, nt = 0;
+};
+
+Options resolve_options(const cs_fused_options *o, const Dag &dag) {
+  Options r;
+  if (o) {
+    r.math_mode = o->math_mode;
+    r.V = o->vector_width;
+    r.U = o->items_per_thread;
+    r.block = o->block_size;
+    r.nt = o->nontemporal_store;
+  }
+  if (r.math_mode != CS_MATH_PRECISE && r.math_mode != CS_MATH_FAST)
+    fail(CS_INVALID_VALUE, "unknown math mode");
+  int64_t last = dag.out_shape.empty() ? 1 : dag.out_shape.back();
+  int64_t total = 1;
+  for (auto s : dag.out_shape) total *= s;
+  if (r.V == 0) r.V = 4;
+  if (r.V != 1 && r.V != 2 && r.V != 4 && r.V != 8) fail(CS_INVALID_VALUE, "vector_width must be 1, 2, 4 or 8");
+  while (r.V > 1 && last % r.V != 0) r.V /= 2;
+  if (r.U == 0) r.U = total >= (int64_t(1) << 22) ? 2 : 1;
+  if (r.U < 1 || r.U > 8) fail(CS_INVALID_VALUE, "items_per_thread must be 1..8");
+  if (r.block == 0) r.block = 256;
+  if (r.block < 64 || r.block > 1024 || r.block % 64 != 0)
+    fail(CS_INVALID_WORK_GROUP_SIZE, "block_size must be a multiple of 64 in 64..1024");
+  return r;
+}
+
+// Rows of a TRANSFORM matrix: the rank of the space its operand lives in.
+int transform_rows(const Dag &dag, const cs_node &n) {
+  const cs_node &opnd = dag.nodes[n.a];
+  return (opnd.op == CS_OP_PARAMETER || opnd.op == CS_OP_TRANSFORM) ? opnd.rank : 0;
+}
+
+std::string canonical_key(const Dag &dag, const Options &o) {
+  std::string k;
+  k.reserve(dag.nodes.size() * 64 + 64);
+  put(k, static_cast<int32_t>(dag.nodes.size()));
+  for (const cs_node &n : dag.nodes) {
+    put(k, n.op);
+    put(k, n.a);
+    put(k, n.b);
+    if (n.op == CS_OP_PARAMETER || n.op == CS_OP_LITERAL) {
+      uint32_t bits;
+      std::memcpy(&bits, &n.value, 4);
+      put(k, bits);
+    }
+    if (n.op == CS_OP_PARAMETER) put(k, n.param);
+    if (n.op == CS_OP_PARAMETER || n.op == CS_OP_TRANSFORM) {
+      put(k, n.rank);
+      for (int i = 0; i < n.rank; ++i) put(k, n.shape[i]);
+    }
+    if (n.op == CS_OP_TRANSFORM) {
+      int rows = transform_rows(dag, n);
+      for (int i = 0; i < rows * (n.rank + 1); ++i) put(k, n.matrix[i]);
+    }
+  }
+  put(k, static_cast<int32_t>(dag.outputs.size()));
+  for (int out : dag.outputs) put(k, out);
+  put(k, static_cast<int32_t>(dag.out_shape.size()));
+  for (auto s : dag.out_shape) put(k, s);
+  put(k, o.math_mode);
+  put(k, o.V);
+  put(k, o.U);
+  put(k, o.block);
+  put(k, o.nt);
+  return k;
+}
+
+// ------------------------------------------------------------------------ code generation
+class Emitter {
+ public:
+  Emitter(const Dag &dag, const Options &o) : dag_(dag), o_(o) {}
+
+  std::string generate(int64_t *vectors_out) {
+    const int r = static_cast<int>(dag_.out_shape.size());
+    const int V = o_.V;
+    int64_t total = 1;
+    for (auto s : dag_.out_shape) total *= s;
+    const int64_t vectors = total / V;
+    *vectors_out = vectors;
+    const int nout = static_cast<int>(dag_.outputs.size());
+
+    std::ostringstream s;
+    s << "// Generated by csmi for gfx950: " << dag_.nodes.size() << " nodes, " << dag_.nleaves
+      << " leaves, output [";
+    for (int i = 0; i < r; ++i) s << (i ? "," : "") << dag_.out_shape[i];
+    s << "] x " << nout << "\n";
+    s << "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
+    s << "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
+    s << "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
+    s << "extern \"C\" __global__ __launch_bounds__(" << o_.block << ") void jit_kernel(";
+    for (int i = 0; i < dag_.nleaves; ++i) s << "const float* __restrict__ p" << i << ", ";
+    s << "float* __restrict__ out) {\n";
+    s << "  const unsigned vbase = blockIdx.x * " << (int64_t(o_.block) * o_.U) << "u + threadIdx.x;\n";
+    s << "#pragma unroll\n";
+    s << "  for (int u = 0; u < " << o_.U << "; ++u) {\n";
+    s << "    const unsigned q = vbase + u * " << o_.block << "u;\n";
+    s << "    if (q >= " << vectors << "u) break;\n";
+    s << "    const unsigned e0 = q * " << V << "u;\n";
+    // Decode the coordinates of the first of the V outputs (row-major, constant shape).
+    if (r > 0) {
+      s << "    unsigned rest = e0;\n";
+      for (int d = r - 1; d >= 0; --d) {
+        if (d > 0) {
+          s << "    const int c" << d << " = (int)(rest % " << dag_.out_shape[d] << "u); rest /= "
+            << dag_.out_shape[d] << "u;\n";
+        } else {
+          s << "    const int c0 = (int)rest;\n";
+        }
+      }
+    }
+    body_ = &s;
+    for (size_t i = 0; i < dag_.nodes.size(); ++i) emit_node(static_cast<int>(i));
+    // Stores: out[(e0+v)*nout + j] is one contiguous run of V*nout floats.
+    const int run = V * nout;
+    s << "    float* __restrict__ dst = out + (unsigned long long)e0 * " << nout << "u;\n";
+    std::vector<std::string> vals;
+    for (int v = 0; v < V; ++v)
+      for (int j = 0; j < nout; ++j) vals.push_back(value_of(dag_.outputs[j], v));
+    int i = 0;
+    const bool aligned16 = (run % 4 == 0);
+    while (i < run) {
+      if (aligned16 && run - i >= 4) {
+        s << "    " << store("dst + " + std::to_string(i),
+                             "csmi_v4{" + vals[i] + ", " + vals[i + 1] + ", " + vals[i + 2] + ", " + vals[i + 3] + "}",
+                             "csmi_v4")
+          << "\n";
+        i += 4;
+      } else {
+        s << "    " << store("dst + " + std::to_string(i), vals[i], "float") << "\n";
+        i += 1;
+      }
+    }
+    s << "  }\n}\n";
+    return s.str();
+  }
+
+ private:
+  std::string store(const std::string &addr, const std::string &val, const std::string &type) {
+    if (o_.nt) return "__builtin_nontemporal_store(" + val + ", (" + type + "*)(" + addr + "));";
+    return "*(" + type + "*)(" + addr + ") = " + val + ";";
+  }
+
+  // Name of the FP32 value of float node `id` for element v.
+  std::string value_of(int id, int v) {
+    const cs_node &n = dag_.nodes[id];
+    if (n.op == CS_OP_LITERAL) return float_literal(n.value);
+    if (uniform_.count(id)) return "t" + std::to_string(id);
+    return "t" + std::to_string(id) + "[" + std::to_string(v) + "]";
+  }
+
+  bool is_uniform(int id) const {
+    const cs_node &n = dag_.nodes[id];
+    return n.op == CS_OP_LITERAL || uniform_.count(id) > 0;
+  }
+
+  void emit_node(int id) {
+    const cs_node &n = dag_.nodes[id];
+    std::ostringstream &s = *body_;
+    const int V = o_.V;
+    if (is_array(n.op) || n.op == CS_OP_LITERAL) return;
+    if (n.op == CS_OP_EXTRACT) {
+      emit_gather(id);
+      return;
+    }
+    // Arithmetic: one SSA statement per operation and element (OpenCLKernelBuilder.scala:516-571,
+    // 271-325). A value that is the same for all V elements is computed once.
+    bool uni = is_uniform(n.a) && (!is_binary(n.op) || is_uniform(n.b));
+    int count = uni ? 1 : V;
+    std::string name = "t" + std::to_string(id);
+    if (uni) {
+      uniform_.insert(id);
+      s << "    float " << name << ";\n";
+    } else {
+      s << "    float " << name << "[" << V << "];\n";
+    }
+    s << "#pragma unroll\n    for (int v = 0; v < " << count << "; ++v) ";
+    auto arg = [&](int child) {
+      const cs_node &c = dag_.nodes[child];
+      if (c.op == CS_OP_LITERAL) return float_literal(c.value);
+      if (uniform_.count(child)) return "t" + std::to_string(child);
+      return "t" + std::to_string(child) + "[v]";
+    };
+    std::string lhs = uni ? name : name + "[v]";
+    std::string a = arg(n.a), b = is_binary(n.op) ? arg(n.b) : "";
+    bool fast = o_.math_mode == CS_MATH_FAST;
+    std::string e;
+    switch (n.op) {
+      case CS_OP_NEG: e = "-" + a; break;
+      case CS_OP_ABS: e = "fabsf(" + a + ")"; break;
+      case CS_OP_SQRT: e = "sqrtf(" + a + ")"; break;
+      case CS_OP_TANH: e = "tanhf(" + a + ")"; break;
+      case CS_OP_EXP: e = (fast ? "__expf(" : "expf(") + a + ")"; break;
+      case CS_OP_LOG: e = (fast ? "__logf(" : "logf(") + a + ")"; break;
+      case CS_OP_ADD: e = a + " + " + b; break;
+      case CS_OP_SUB: e = a + " - " + b; break;
+      case CS_OP_MUL: e = a + " * " + b; break;
+      case CS_OP_DIV: e = a + " / " + b; break;
+      // OpenCL C has no float `%` (the reference's text fails to build, Appendix quirk);
+      // the C fmod is the meaning the operator has everywhere else.
+      case CS_OP_MOD: e = "fmodf(" + a + ", " + b + ")"; break;
+      case CS_OP_MIN: e = "fminf(" + a + ", " + b + ")"; break;
+      case CS_OP_MAX: e = "fmaxf(" + a + ", " + b + ")"; break;
+      default: fail(CS_INVALID_VALUE, "unexpected op in codegen");
+    }
+    s << lhs << " = " << e << ";\n";
+  }
+
+  // A leaf read at the current output coordinates (OpenCLKernelBuilder.scala:348-411 for
+  // views, 435-455 for a bare parameter).
+  void emit_gather(int id) {
+    std::ostringstream &s = *body_;
+    const int V = o_.V;
+    const int r = static_cast<int>(dag_.out_shape.size());
+    const cs_node &n = dag_.nodes[id];
+    ArrayRef ar = resolve_array(dag_, n.a);
+    std::string name = "t" + std::to_string(id);
+    if (ar.is_fill) {
+      // extract(fill(x)) == x
+      const cs_node &fv = dag_.nodes[ar.fill_value];
+      if (fv.op == CS_OP_LITERAL || uniform_.count(ar.fill_value)) {
+        uniform_.insert(id);
+        s << "    const float " << name << " = " << value_of(ar.fill_value, 0) << ";\n";
+      } else {
+        s << "    float " << name << "[" << V << "];\n#pragma unroll\n    for (int v = 0; v < " << V
+          << "; ++v) " << name << "[v] = t" << ar.fill_value << "[v];\n";
+      }
+      return;
+    }
+    const cs_node &p = dag_.nodes[ar.param];
+    const int src_rank = p.rank;
+    const std::string ptr = "p" + std::to_string(p.param);
+    const std::string pad = float_literal(p.value);
+    if (src_rank == 0) {
+      // `*x`: a scalar source is read without bounds (OpenCLKernelBuilder.scala:351-356).
+      uniform_.insert(id);
+      s << "    const float " << name << " = " << ptr << "[0];\n";
+      return;
+    }
+    // Row y of the index map: idx_y = (int)(sum_x m[y][x] * c_x + m[y][r]).
+    struct Row {
+      bool empty = true;        // all-zero row: index 0, no bounds check
+      bool fractional = false;  // double arithmetic, truncated by (int)
+      std::string expr;         // index of element 0 (or the per-element expression)
+      int64_t inner_step = 0;   // integer coefficient of the innermost output coordinate
+      bool upper_only = false;  // bare parameter: only idx < dim is checked
+    };
+    std::vector<Row> rows(static_cast<size_t>(src_rank));
+    if (!ar.has_view) {
+      // Bare parameter: (*x)[gid_0]...[gid_{rank-1}] with upper bounds only.
+      if (src_rank > r)
+        fail(CS_INVALID_VALUE, "a leaf of rank " + std::to_string(src_rank) +
+                                   " cannot be read directly in a rank-" + std::to_string(r) + " kernel");
+      for (int y = 0; y < src_rank; ++y) {
+        rows[y].empty = false;
+        rows[y].upper_only = true;
+        rows[y].expr = "c" + std::to_string(y);
+        rows[y].inner_step = (y == r - 1) ? 1 : 0;
+      }
+    } else {
+      if (ar.out_rank != r)
+        fail(CS_INVALID_VALUE, "a view of rank " + std::to_string(ar.out_rank) + " is read in a rank-" +
+                                   std::to_string(r) + " kernel");
+      const int cols = r + 1;
+      for (int y = 0; y < src_rank; ++y) {
+        Row &row = rows[y];
+        std::vector<std::string> terms;
+        bool frac = false;
+        for (int x = 0; x < cols; ++x) {
+          double m = ar.matrix[static_cast<size_t>(y) * cols + x];
+          if (m == 0.0) continue;
+          std::string lit = decimal_format(m);
+          if (lit.find('.') != std::string::npos) frac = true;
+          if (x < cols - 1) {
+            if (m == 1.0) terms.push_back("c" + std::to_string(x));
+            else terms.push_back("c" + std::to_string(x) + " * " + lit);
+            if (x == r - 1) {
+              if (lit.find('.') == std::string::npos) row.inner_step = std::stoll(lit);
+            }
+          } else {
+            terms.push_back(lit);
+          }
+        }
+        if (terms.empty()) continue;
+        row.empty = false;
+        row.fractional = frac;
+        std::string e;
+        for (size_t t = 0; t < terms.size(); ++t) e += (t ? " + " : "") + terms[t];
+        row.expr = e;
+      }
+    }
+    // Source row-major strides.
+    std::vector<int64_t> stride(static_cast<size_t>(src_rank), 1);
+    for (int y = src_rank - 2; y >= 0; --y) stride[y] = stride[y + 1] * p.shape[y + 1];
+
+    bool any_frac = false;
+    for (auto &row : rows) any_frac = any_frac || row.fractional;
+    // Classify the V elements: broadcast (no row depends on the innermost coordinate),
+    // contiguous (only the innermost source row moves, by +1), or general.
+    bool inner_static = true;
+    int moving = -1, nmoving = 0;
+    for (int y = 0; y < src_rank; ++y) {
+      if (rows[y].empty) continue;
+      if (rows[y].fractional) { inner_static = false; continue; }
+      if (rows[y].inner_step != 0) { moving = y; ++nmoving; }
+    }
+    const bool broadcast = !any_frac && nmoving == 0;
+    const bool contiguous = !any_frac && nmoving == 1 && moving == src_rank - 1 &&
+                            rows[moving].inner_step == 1 && V > 1;
+    (void)inner_static;
+
+    const std::string g = "g" + std::to_string(id);
+    // Per-row index of element 0 and the combined bounds predicate.
+    auto emit_rows = [&](const std::string &suffix, const std::string &cinner) {
+      std::vector<std::string> conds;
+      std::string off;
+      for (int y = 0; y < src_rank; ++y) {
+        const Row &row = rows[y];
+        std::string ix = g + "_i" + std::to_string(y) + suffix;
+        if (row.empty) {
+          s << "      const int " << ix << " = 0;\n";
+        } else {
+          std::string e = row.expr;
+          if (!cinner.empty() && r > 0) {
+            // replace the innermost coordinate name with the per-element one
+            std::string from = "c" + std::to_string(r - 1);
+            std::string out;
+            size_t pos = 0;
+            while (pos < e.size()) {
+              if (e.compare(pos, from.size(), from) == 0 &&
+                  (pos + from.size() == e.size() || !isdigit(static_cast<unsigned char>(e[pos + from.size()])))) {
+                out += cinner;
+                pos += from.size();
+              } else {
+                out += e[pos++];
+              }
+            }
+            e = out;
+          }
+          s << "      const int " << ix << " = (int)(" << e << ");\n";
+          if (row.upper_only) conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+          else {
+            conds.push_back(ix + " >= 0");
+            conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+          }
+        }
+        std::string term = stride[y] == 1 ? ix : ix + " * " + std::to_string(stride[y]);
+        off += (off.empty() ? "" : " + ") + term;
+      }
+      std::string cond;
+      for (size_t c = 0; c < conds.size(); ++c) cond += (c ? " && " : "") + conds[c];
+      return std::make_pair(off, cond);
+    };
+
+    if (broadcast || V == 1) {
+      s << "    float " << name << "_s;\n    {\n";
+      auto oc = emit_rows("", "");
+      if (oc.second.empty())
+        s << "      " << name << "_s = " << ptr << "[" << oc.first << "];\n";
+      else
+        s << "      " << name << "_s = (" << oc.second << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
+      s << "    }\n";
+      if (V == 1) {
+        s << "    const float " << name << "[1] = {" << name << "_s};\n";
+      } else {
+        uniform_.insert(id);
+        s << "    const float " << name << " = " << name << "_s;\n";
+      }
+      return;
+    }
+    s << "    float " << name << "[" << V << "];\n";
+    if (contiguous) {
+      // One wide load when all V elements are inside the source row, else per element.
+      s << "    {\n";
+      auto oc = emit_rows("", "");
+      std::string yi = g + "_i" + std::to_string(moving);
+      // Bounds of element 0 on the other rows, and of elements 0..V-1 on the moving row.
+      std::vector<std::string> conds;
+      for (int y = 0; y < src_rank; ++y) {
+        if (rows[y].empty || y == moving) continue;
+        std::string ix = g + "_i" + std::to_string(y);
+        if (rows[y].upper_only) conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+        else { conds.push_back(ix + " >= 0"); conds.push_back(ix + " < " + std::to_string(p.shape[y])); }
+      }
+      std::string others;
+      for (size_t c = 0; c < conds.size(); ++c) others += (c ? " && " : "") + conds[c];
+      if (others.empty()) others = "true";
+      s << "      const bool " << g << "_ok = " << others << ";\n";
+      s << "      const float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
+      s << "      if (" << g << "_ok && " << yi << " >= 0 && " << yi << " + " << (V - 1) << " < " << p.shape[moving] << ") {\n";
+      if (V == 8) {
+        s << "        const csmi_f4 a_ = *(const csmi_f4*)(" << g << "_p);\n";
+        s << "        const csmi_f4 b_ = *(const csmi_f4*)(" << g << "_p + 4);\n";
+        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
+        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v + 4 << "] = b_[" << v << "];\n";
+      } else if (V == 4) {
+        s << "        const csmi_f4 a_ = *(const csmi_f4*)(" << g << "_p);\n";
+        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
+      } else {
+        s << "        const csmi_f2 a_ = *(const csmi_f2*)(" << g << "_p);\n";
+        for (int v = 0; v < 2; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
+      }
+      s << "      } else {\n#pragma unroll\n        for (int v = 0; v < " << V << "; ++v) {\n";
+      s << "          const int j_ = " << yi << " + v;\n";
+      s << "          " << name << "[v] = (" << g << "_ok && j_ >= 0 && j_ < " << p.shape[moving] << ") ? "
+        << g << "_p[v] : " << pad << ";\n";
+      s << "        }\n      }\n    }\n";
+      return;
+    }
+    // General gather: every element computes its own index.
+    s << "#pragma unroll\n    for (int v = 0; v < " << V << "; ++v) {\n";
+    std::string cin = "(c" + std::to_string(r - 1) + " + v)";
+    auto oc = emit_rows("", cin);
+    if (oc.second.empty())
+      s << "      " << name << "[v] = " << ptr << "[" << oc.first << "];\n";
+    else
+      s << "      " << name << "[v] = (" << oc.second << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
+    s << "    }\n";
+  }
+
+  const Dag &dag_;
+  Options o_;
+  std::ostringstream *body_ = nullptr;
+  std::set<int> uniform_;
+};
+
+std::vector<std::string> math_options(int mode) {
+  if (mode == CS_MATH_FAST) return {"-ffast-math", "-ffp-contract=fast"};
+  return {"-ffp-contract=off", "-fno-fast-math"};
+}
+
+}  // namespace
+}  // namespace csmi
+
+using namespace csmi;
+
+extern "C" {
+
+cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32_t *outputs,
+                                   int noutputs, int out_rank, const int32_t *out_shape,
+                                   const cs_fused_options *options, char **source, uint64_t *key) {
+  return guard([&] {
+    if (!source) fail(CS_INVALID_VALUE, "source is null");
+    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
+    Options o = resolve_options(options, dag);
+    int64_t vectors = 0;
+    std::string src = Emitter(dag, o).generate(&vectors);
+    *source = static_cast<char *>(std::malloc(src.size() + 1));
+    if (!*source) fail(CS_OUT_OF_HOST_MEMORY, "malloc");
+    std::memcpy(*source, src.c_str(), src.size() + 1);
+    if (key) *key = fnv1a(canonical_key(dag, o));
+  });
+}
+
+cs_status cs_program_check(const char *source, const char *options, char **log) {
+  if (log) *log = nullptr;
+  std::string l;
+  cs_status s = guard([&] {
+    if (!source) fail(CS_INVALID_VALUE, "source is null");
+    std::vector<std::string> opts;
+    if (options) {
+      std::istringstream in(options);
+      std::string w;
+      while (in >> w) opts.push_back(w);
+    }
+    size_t size = 0;
+    if (!compile_check(source, opts, "gfx950", &l, &size))
+      fail(CS_BUILD_PROGRAM_FAILURE, "hiprtc build failed:\n" + l);
+  });
+  if (log && !l.empty()) {
+    *log = static_cast<char *>(std::malloc(l.size() + 1));
+    if (*log) std::memcpy(*log, l.c_str(), l.size() + 1);
+  }
+  return s;
+}
+
+cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
+                           const int32_t *outputs, int noutputs, int out_rank,
+                           const int32_t *out_shape, const cs_fused_options *options,
+                           cs_fused **fused) {
+  return guard([&] {
+    if (!device) fail(CS_INVALID_DEVICE, "device is null");
+    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
+    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
+    Options o = resolve_options(options, dag);
+    std::string key = canonical_key(dag, o);
+    {
+      std::lock_guard<std::recursive_mutex> l(device->mu);
+      auto it = device->fused_cache.find(key);
+      if (it != device->fused_cache.end()) {
+        it->second->refs++;
+        *fused = it->second;
+        return;
+      }
+    }
+    int64_t vectors = 0;
+    std::string src = Emitter(dag, o).generate(&vectors);
+    std::string log;
+    hipModule_t m = build_module(device, src, math_options(o.math_mode), &log);
+    auto *f = new cs_fused();
+    f->device = device;
+    f->key = key;
+    f->hash = fnv1a(key);
+    f->source = src;
+    f->module = m;
+    CS_HIP(hipModuleGetFunction(&f->function, m, "jit_kernel"));
+    f->nleaves = dag.nleaves;
+    int64_t total = 1;
+    for (auto s : dag.out_shape) total *= s;
+    f->out_elements = total * static_cast<int64_t>(dag.outputs.size());
+    f->vectors = vectors;
+    f->block = static_cast<uint32_t>(o.block);
+    int64_t per_block = int64_t(o.block) * o.U;
+    f->grid = static_cast<uint32_t>((vectors + per_block - 1) / per_block);
+    std::lock_guard<std::recursive_mutex> l(device->mu);
+    auto it = device->fused_cache.find(key);
+    if (it != device->fused_cache.end()) {  // another thread won the race
+      (void)hipModuleUnload(m);
+      delete f;
+      it->second->refs++;
+      *fused = it->second;
+      return;
+    }
+    f->refs = 2;  // one for the cache, one for the caller
+    device->fused_cache.emplace(key, f);
+    *fused = f;
+  });
+}
+
+cs_status cs_fused_retain(cs_fused *fused) {
+  return guard([&] {
+    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
+    fused->refs++;
+  });
+}
+
+cs_status cs_fused_release(cs_fused *fused) {
+  return guard([&] {
+    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
+    if (--fused->refs == 0) {
+      set_device(fused->device);
+      (void)hipModuleUnload(fused->module);
+      delete fused;
+    }
+  });
+}
+
+cs_status cs_fused_source(cs_fused *fused, const char **source) {
+  return guard([&] {
+    if (!fused || !source) fail(CS_INVALID_VALUE, "null argument");
+    *source = fused->source.c_str();
+  });
+}
+
+cs_status cs_fused_signature(cs_fused *fused, int *nleaves, int64_t *out_elements) {
+  return guard([&] {
+    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
+    if (nleaves) *nleaves = fused->nleaves;
+    if (out_elements) *out_elements = fused->out_elements;
+  });
+}
+
+cs_status cs_fused_key(cs_fused *fused, uint64_t *key) {
+  return guard([&] {
+    if (!fused || !key) fail(CS_INVALID_VALUE, "null argument");
+    *key = fused->hash;
+  });
+}
+
+cs_status cs_fused_launch(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
+                          void *out) {
+  return guard([&] {
+    if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (nleaves != fused->nleaves)
+      fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
+                                       std::to_string(nleaves));
+    if (fused->grid == 0) return;
+    if (!out) fail(CS_INVALID_KERNEL_ARGS, "output pointer is null");
+    std::vector<const void *> vals(static_cast<size_t>(nleaves) + 1);
+    std::vector<void *> args(static_cast<size_t>(nleaves) + 1);
+    for (int i = 0; i < nleaves; ++i) {
+      if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
+      vals[i] = leaves[i];
+      args[i] = &vals[i];
+    }
+    vals[nleaves] = out;
+    args[nleaves] = &vals[nleaves];
+    set_device(stream->device);
+    CS_HIP(hipModuleLaunchKernel(fused->function, fused->grid, 1, 1, fused->block, 1, 1, 0,
+                                 stream->stream, args.data(), nullptr));
+  });
+}
+
+cs_status cs_fused_lookup(cs_device *device, const cs_node *nodes, int nnodes, const int32_t *outputs,
+                          int noutputs, int out_rank, const int32_t *out_shape,
+                          const cs_fused_options *options, int *hit) {
+  return guard([&] {
+    if (!device || !hit) fail(CS_INVALID_VALUE, "null argument");
+    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
+    Options o = resolve_options(options, dag);
+    std::string key = canonical_key(dag, o);
+    std::lock_guard<std::recursive_mutex> l(device->mu);
+    *hit = device->fused_cache.count(key) ? 1 : 0;
+  });
+}
+
+cs_status cs_fused_cache_clear(cs_device *device) {
+  return guard([&] {
+    if (!device) fail(CS_INVALID_DEVICE, "device is null");
+    std::vector<cs_fused *> drop;
+    {
+      std::lock_guard<std::recursive_mutex> l(device->mu);
+      for (auto &kv : device->fused_cache) drop.push_back(kv.second);
+      device->fused_cache.clear();
+    }
+    for (cs_fused *f : drop) cs_fused_release(f);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,382 @@
+// Reductions for gfx950: full sum and max, row-wise sum/max/total in one pass, column sums.
+//
+// Replaces the reference's `sum` programs (Tensors.scala:308-393, 673-771). Layout and
+// arithmetic:
+//   * 256-thread workgroups (4 wave64s); each thread reads 16-byte vectors, several in
+//     flight, so a row of 4096 floats is 4 loads per thread;
+//   * per-thread sums are FP32 with a TwoSum compensation term, combined across the wave
+//     with xor shuffles and across the 4 waves through LDS, still as (sum, compensation)
+//     pairs, so the in-block result is accurate to about one FP32 rounding;
+//   * block results are combined in FP64: every block writes an FP64 partial, and the last
+//     block to finish (a ticket counter the last block resets to zero) adds the partials
+//     in a fixed order, so results are deterministic run to run;
+//   * max uses fmaxf, which returns the non-NaN operand (OpenCLKernelBuilder.scala:279-285).
+// The reference's parallel program loses elements when the local size is not a power of
+// two (Tensors.scala:375-387); these kernels sum every element.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "internal.h"
+
+#pragma clang fp contract(off)
+
+namespace csmi {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
+  s = a + b;
+  float bp = s - a;
+  e = (a - (s - bp)) + (b - bp);
+}
+
+struct Acc {
+  float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ void add(float x) {
+    float t, e;
+    two_sum(s, x, t, e);
+    s = t;
+    c += e;
+  }
+  __device__ __forceinline__ void merge(float s2, float c2) {
+    float t, e;
+    two_sum(s, s2, t, e);
+    s = t;
+    c += c2 + e;
+  }
+};
+
+__device__ __forceinline__ float nanmax(float a, float b) { return fmaxf(a, b); }
+
+// Sum pair and max across the 64 lanes of a wave.
+__device__ __forceinline__ void wave_reduce(Acc &a, float &m) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    float s2 = __shfl_xor(a.s, off, 64);
+    float c2 = __shfl_xor(a.c, off, 64);
+    float m2 = __shfl_xor(m, off, 64);
+    a.merge(s2, c2);
+    m = nanmax(m, m2);
+  }
+}
+
+// Reduces (a, m) over the block; the result is valid in thread 0.
+__device__ __forceinline__ void block_reduce(Acc &a, float &m) {
+  __shared__ float ls[kWaves], lc[kWaves], lm[kWaves];
+  wave_reduce(a, m);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    ls[wave] = a.s;
+    lc[wave] = a.c;
+    lm[wave] = m;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    Acc t;
+    t.s = ls[0];
+    t.c = lc[0];
+    float mm = lm[0];
+#pragma unroll
+    for (int w = 1; w < kWaves; ++w) {
+      t.merge(ls[w], lc[w]);
+      mm = nanmax(mm, lm[w]);
+    }
+    a = t;
+    m = mm;
+  }
+}
+
+// FP64 sum of partials[0..n) by one block in a fixed order; result in thread 0.
+__device__ double block_sum64(const double *partials, int64_t n) {
+  __shared__ double l64[kBlock];
+  double acc = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) acc += __builtin_nontemporal_load(partials + i);
+  l64[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) l64[threadIdx.x] += l64[threadIdx.x + off];
+    __syncthreads();
+  }
+  return l64[0];
+}
+
+__device__ float block_max_partials(const float *partials, int64_t n) {
+  __shared__ float lmx[kBlock];
+  float m = __builtin_nanf("");
+  for (int64_t i = threadIdx.x; i < n; i += kBlock) m = nanmax(m, __builtin_nontemporal_load(partials + i));
+  lmx[threadIdx.x] = m;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lmx[threadIdx.x] = nanmax(lmx[threadIdx.x], lmx[threadIdx.x + off]);
+    __syncthreads();
+  }
+  return lmx[0];
+}
+
+// True in every thread of the last block to arrive. Partial results written before the
+// call are visible to that block afterwards. The last block resets the ticket.
+__device__ bool last_block(unsigned *ticket, unsigned nblocks) {
+  __shared__ bool last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(ticket, 1u);
+    last = (prev == nblocks - 1);
+    if (last) atomicExch(ticket, 0u);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+// ----------------------------------------------------------------------------- full sum/max
+template <bool kMax>
+__global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restrict__ in, int64_t n,
+                                                            double *partial64, float *partialmax,
+                                                            unsigned *ticket, float *out,
+                                                            double *out64) {
+  Acc a;
+  float m = __builtin_nanf("");
+  const int64_t nvec = ((reinterpret_cast<uintptr_t>(in) & 15) == 0) ? n / 4 : 0;
+  const f4 *in4 = reinterpret_cast<const f4 *>(in);
+  // Block-strided chunks of kBlock vectors: consecutive threads read consecutive vectors.
+  const int64_t stride = int64_t(gridDim.x) * kBlock;
+  int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  for (; i + 3 * stride < nvec; i += 4 * stride) {
+    f4 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = __builtin_nontemporal_load(in4 + i + k * stride);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      if (kMax) {
+        m = nanmax(m, nanmax(nanmax(v[k].x, v[k].y), nanmax(v[k].z, v[k].w)));
+      } else {
+        a.add(v[k].x); a.add(v[k].y); a.add(v[k].z); a.add(v[k].w);
+      }
+    }
+  }
+  for (; i < nvec; i += stride) {
+    f4 v = __builtin_nontemporal_load(in4 + i);
+    if (kMax) {
+      m = nanmax(m, nanmax(nanmax(v.x, v.y), nanmax(v.z, v.w)));
+    } else {
+      a.add(v.x); a.add(v.y); a.add(v.z); a.add(v.w);
+    }
+  }
+  for (int64_t j = nvec * 4 + int64_t(blockIdx.x) * kBlock + threadIdx.x; j < n; j += stride) {
+    float x = in[j];
+    if (kMax) m = nanmax(m, x); else a.add(x);
+  }
+  block_reduce(a, m);
+  if (threadIdx.x == 0) {
+    if (kMax) partialmax[blockIdx.x] = m;
+    else partial64[blockIdx.x] = double(a.s) + double(a.c);
+  }
+  if (!last_block(ticket, gridDim.x)) return;
+  if (kMax) {
+    float r = block_max_partials(partialmax, gridDim.x);
+    if (threadIdx.x == 0) *out = r;
+  } else {
+    double r = block_sum64(partial64, gridDim.x);
+    if (threadIdx.x == 0) {
+      if (out) *out = float(r);
+      if (out64) *out64 = r;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------- rows
+// One workgroup per row: row sum (compensated FP32, rounded once), row max, and the FP64
+// partial of the row for the total.
+template <bool kVec>
+__global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__restrict__ in, int64_t rows,
+                                                             int64_t cols, float *row_sum, float *row_max,
+                                                             double *partial64, unsigned *ticket,
+                                                             float *total, double *total64) {
+  const int64_t row = blockIdx.x;
+  const float *p = in + row * cols;
+  Acc a;
+  float m = __builtin_nanf("");
+  if (kVec) {
+    const f4 *p4 = reinterpret_cast<const f4 *>(p);
+    const int64_t nv = cols / 4;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * kBlock < nv; i += 4 * kBlock) {
+      f4 v[4];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) v[k] = __builtin_nontemporal_load(p4 + i + k * kBlock);
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        Acc t;  // pairwise inside the vector, then one compensated add
+        float s01, e01, s23, e23, s, e;
+        two_sum(v[k].x, v[k].y, s01, e01);
+        two_sum(v[k].z, v[k].w, s23, e23);
+        two_sum(s01, s23, s, e);
+        a.merge(s, e + e01 + e23);
+        m = nanmax(m, nanmax(nanmax(v[k].x, v[k].y), nanmax(v[k].z, v[k].w)));
+      }
+    }
+    for (; i < nv; i += kBlock) {
+      f4 v = __builtin_nontemporal_load(p4 + i);
+      a.add(v.x); a.add(v.y); a.add(v.z); a.add(v.w);
+      m = nanmax(m, nanmax(nanmax(v.x, v.y), nanmax(v.z, v.w)));
+    }
+  } else {
+    for (int64_t i = threadIdx.x; i < cols; i += kBlock) {
+      float x = p[i];
+      a.add(x);
+      m = nanmax(m, x);
+    }
+  }
+  block_reduce(a, m);
+  if (threadIdx.x == 0) {
+    if (row_sum) row_sum[row] = a.s + a.c;
+    if (row_max) row_max[row] = m;
+    if (partial64) partial64[row] = double(a.s) + double(a.c);
+  }
+  if (!partial64) return;
+  if (!last_block(ticket, gridDim.x)) return;
+  double r = block_sum64(partial64, rows);
+  if (threadIdx.x == 0) {
+    if (total) *total = float(r);
+    if (total64) *total64 = r;
+  }
+}
+
+// ----------------------------------------------------------------------------- columns
+// Column sums: thread = column, blockIdx.y = a slice of rows; FP64 partials per slice; the
+// last slice block of each column group adds the slices in order.
+__global__ __launch_bounds__(kBlock) void reduce_cols_kernel(const float *__restrict__ in, int64_t rows,
+                                                             int64_t cols, int64_t rows_per_slice,
+                                                             double *partial64, unsigned *tickets,
+                                                             float *col_sum) {
+  const int64_t col = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  const int64_t r0 = int64_t(blockIdx.y) * rows_per_slice;
+  const int64_t r1 = r0 + rows_per_slice < rows ? r0 + rows_per_slice : rows;
+  Acc a;
+  if (col < cols)
+    for (int64_t r = r0; r < r1; ++r) a.add(__builtin_nontemporal_load(in + r * cols + col));
+  if (col < cols) partial64[int64_t(blockIdx.y) * cols + col] = double(a.s) + double(a.c);
+  if (!last_block(tickets + blockIdx.x, gridDim.y)) return;
+  if (col < cols) {
+    double s = 0.0;
+    for (unsigned y = 0; y < gridDim.y; ++y) s += __builtin_nontemporal_load(partial64 + int64_t(y) * cols + col);
+    col_sum[col] = float(s);
+  }
+}
+
+}  // namespace
+
+// Per-stream scratch: [0, 4 KiB) tickets (zeroed at allocation, reset by the kernels),
+// then FP64/FP32 partials.
+struct ReduceScratch {
+  unsigned *tickets;
+  double *partial64;
+  float *partialmax;
+};
+
+}  // namespace csmi
+
+using namespace csmi;
+
+static ReduceScratch scratch_for(cs_stream *stream, size_t partial_doubles, size_t tickets) {
+  const size_t ticket_bytes = ((tickets * sizeof(unsigned) + 4095) / 4096) * 4096;
+  char *base = static_cast<char *>(stream_scratch(stream, ticket_bytes + partial_doubles * 8 + 4096));
+  ReduceScratch r;
+  r.tickets = reinterpret_cast<unsigned *>(base);
+  r.partial64 = reinterpret_cast<double *>(base + ticket_bytes);
+  r.partialmax = reinterpret_cast<float *>(r.partial64);
+  return r;
+}
+
+static int reduce_grid(cs_stream *stream, int64_t n) {
+  int cus = stream->device->props.multiProcessorCount;
+  int64_t want = (n / 4 + kBlock * 4 - 1) / (kBlock * 4);  // >= 4 vectors per thread
+  int64_t cap = int64_t(cus) * 8;
+  if (want > cap) want = cap;
+  if (want < 1) want = 1;
+  return static_cast<int>(want);
+}
+
+extern "C" {
+
+cs_status cs_reduce_sum(cs_stream *stream, const float *in, int64_t n, float *out, double *out64) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (n < 0) fail(CS_INVALID_VALUE, "negative length");
+    if (n > 0 && !in) fail(CS_INVALID_VALUE, "input is null");
+    if (!out && !out64) fail(CS_INVALID_VALUE, "no output");
+    set_device(stream->device);
+    int grid = reduce_grid(stream, n);
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid), 1);
+    hipLaunchKernelGGL(reduce_all_kernel<false>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
+                       s.partial64, s.partialmax, s.tickets, out, out64);
+    CS_HIP(hipGetLastError());
+  });
+}
+
+cs_status cs_reduce_max(cs_stream *stream, const float *in, int64_t n, float *out) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (n < 0) fail(CS_INVALID_VALUE, "negative length");
+    if (n > 0 && !in) fail(CS_INVALID_VALUE, "input is null");
+    if (!out) fail(CS_INVALID_VALUE, "output is null");
+    set_device(stream->device);
+    int grid = reduce_grid(stream, n);
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid), 1);
+    hipLaunchKernelGGL(reduce_all_kernel<true>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
+                       s.partial64, s.partialmax, s.tickets, out, nullptr);
+    CS_HIP(hipGetLastError());
+  });
+}
+
+cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64_t cols, float *row_sum,
+                         float *row_max, float *total, double *total64) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (rows < 0 || cols < 0) fail(CS_INVALID_VALUE, "negative shape");
+    if (rows > 0x7fffffff) fail(CS_INVALID_VALUE, "more than 2^31-1 rows");
+    if (rows > 0 && cols > 0 && !in) fail(CS_INVALID_VALUE, "input is null");
+    if (rows == 0) return;
+    set_device(stream->device);
+    bool want_total = total || total64;
+    ReduceScratch s = scratch_for(stream, want_total ? static_cast<size_t>(rows) : 0, 1);
+    bool vec = cols % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    auto k = vec ? reduce_rows_kernel<true> : reduce_rows_kernel<false>;
+    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(rows)), dim3(kBlock), 0, stream->stream, in, rows, cols,
+                       row_sum, row_max, want_total ? s.partial64 : nullptr, s.tickets, total, total64);
+    CS_HIP(hipGetLastError());
+  });
+}
+
+cs_status cs_reduce_cols(cs_stream *stream, const float *in, int64_t rows, int64_t cols, float *col_sum) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (rows < 0 || cols < 0) fail(CS_INVALID_VALUE, "negative shape");
+    if (cols == 0) return;
+    if (!col_sum || (rows > 0 && !in)) fail(CS_INVALID_VALUE, "null pointer");
+    set_device(stream->device);
+    int64_t groups = (cols + kBlock - 1) / kBlock;
+    if (groups > 1024) fail(CS_INVALID_VALUE, "more than 262144 columns");
+    int cus = stream->device->props.multiProcessorCount;
+    int64_t slices = (int64_t(cus) * 4 + groups - 1) / groups;
+    int64_t max_slices = (rows + 63) / 64;
+    if (slices > max_slices) slices = max_slices;
+    if (slices < 1) slices = 1;
+    if (slices > 65535) slices = 65535;
+    int64_t per = (rows + slices - 1) / slices;
+    if (per < 1) per = 1;
+    slices = rows > 0 ? (rows + per - 1) / per : 1;
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(slices * cols), static_cast<size_t>(groups));
+    hipLaunchKernelGGL(reduce_cols_kernel, dim3(static_cast<unsigned>(groups), static_cast<unsigned>(slices)),
+                       dim3(kBlock), 0, stream->stream, in, rows, cols, per, s.partial64, s.tickets, col_sum);
+    CS_HIP(hipGetLastError());
+  });
+}
+
+}  // extern "C"

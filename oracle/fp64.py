@@ -1,0 +1,205 @@
+"""numpy oracle for fused expressions, reductions and GEMM (test infrastructure).
+
+`evaluate(closure, shape, leaves, mode)` interprets a closure DAG (the facade's node
+objects, duck-typed by class name) over the output coordinates with numpy:
+
+  mode "fp64":  every operation in FP64 on the FP32 inputs;
+  mode "fp32":  every operation's result rounded to FP32 (+ - * / fmod min max exactly
+                as IEEE FP32; exp, log, tanh, sqrt computed in FP64 and rounded once),
+                i.e. the FP32 program with correctly rounded operations.
+
+The view semantics are the reference's (OpenCLKernelBuilder.scala:348-411, 435-455):
+`idx = (int)(sum of coordinate * coefficient + constant)`, truncated toward zero, with
+each coefficient rounded to three decimals as the kernel text prints it
+(OpenCLKernelBuilder.scala:14-32); a view's source index outside [0, dim) reads the
+padding; an all-zero row reads index 0 unchecked; a bare leaf checks only idx < dim.
+View chains compose as homogeneous matrix products (the algebra of
+NDimensionalAffineTransform.scala:28-95), written independently of the product code.
+"""
+import decimal
+
+import numpy as np
+
+
+def _round3(v):
+    return float(decimal.Decimal(float(v)).quantize(decimal.Decimal("0.001"), rounding=decimal.ROUND_HALF_EVEN))
+
+
+def _homogeneous(matrix, rows, cols_in):
+    m = np.zeros((rows + 1, cols_in + 1))
+    m[:rows, :] = np.asarray(matrix, dtype=np.float64).reshape(rows, cols_in + 1)
+    m[rows, cols_in] = 1.0
+    return m
+
+
+def _resolve(node):
+    """(kind, payload): ("fill", float node) or ("array", param, matrix or None)."""
+    kind = type(node).__name__
+    if kind == "ArrayParameter":
+        return ("array", node, None)
+    if kind == "Fill":
+        return ("fill", node.element)
+    if kind == "Transform":
+        inner = _resolve(node.array)
+        if inner[0] == "fill":
+            return inner
+        _, param, m = inner
+        src_rank = len(param.shape)
+        view_in = len(node.matrix) // (node.rank + 1)  # rows = rank of the inner space
+        mine = _homogeneous(node.matrix, view_in, node.rank)
+        if m is None:
+            return ("array", param, mine)
+        return ("array", param, m @ mine)  # inner map after this view
+    raise TypeError("not an array node: " + kind)
+
+
+def _gather(param, matrix, data, out_shape):
+    src_shape = tuple(param.shape)
+    data = np.asarray(data, dtype=np.float32).reshape(src_shape) if src_shape else np.asarray(data, np.float32).ravel()
+    pad = np.float32(param.padding)
+    r = len(out_shape)
+    coords = np.indices(out_shape, dtype=np.int64) if r else np.zeros((0,), np.int64)
+    if len(src_shape) == 0:
+        return np.full(out_shape, data.ravel()[0], dtype=np.float32)
+    ok = np.ones(out_shape, dtype=bool)
+    idx = []
+    for y in range(len(src_shape)):
+        if matrix is None:
+            if y >= r:
+                raise ValueError("leaf rank exceeds the kernel rank")
+            iy = coords[y]
+            ok &= iy < src_shape[y]
+        else:
+            row = matrix[y]
+            coefs = [(x, _round3(row[x])) for x in range(r) if row[x] != 0.0]
+            const = _round3(row[r]) if row[r] != 0.0 else None
+            if not coefs and const is None:
+                iy = np.zeros(out_shape, dtype=np.int64)
+            else:
+                if all(float(c).is_integer() for _, c in coefs) and (const is None or float(const).is_integer()):
+                    acc = np.zeros(out_shape, dtype=np.int64)
+                    for x, c in coefs:
+                        acc = acc + coords[x] * int(c)
+                    if const is not None:
+                        acc = acc + int(const)
+                    iy = acc
+                else:
+                    acc = np.zeros(out_shape, dtype=np.float64)
+                    for x, c in coefs:
+                        acc = acc + coords[x] * c
+                    if const is not None:
+                        acc = acc + const
+                    iy = np.trunc(acc).astype(np.int64)
+                ok &= (iy >= 0) & (iy < src_shape[y])
+        idx.append(iy)
+    safe = [np.where(ok, iy, 0) for iy in idx]
+    vals = data[tuple(safe)]
+    return np.where(ok, vals, pad).astype(np.float32)
+
+
+def evaluate(closure, shape, leaves, mode="fp64"):
+    """Values of `closure` at every coordinate of `shape` (tuples: last axis = element).
+    `leaves` maps each ArrayParameter (by identity) or leaf index to its data."""
+    shape = tuple(int(s) for s in shape)
+    f32 = mode == "fp32"
+    dt = np.float32 if f32 else np.float64
+    memo = {}
+    params = _param_order(closure)
+    data_of = {}
+    for i, p in enumerate(params):
+        data_of[id(p)] = leaves[i] if isinstance(leaves, (list, tuple)) else leaves[id(p)]
+
+    def ev(node):
+        key = id(node)
+        if key in memo:
+            return memo[key]
+        kind = type(node).__name__
+        if kind == "FloatLiteral":
+            v = np.full(shape, node.value, dtype=np.float32).astype(dt)
+        elif kind == "Extract":
+            res = _resolve(node.array)
+            if res[0] == "fill":
+                v = ev(res[1])
+            else:
+                _, param, m = res
+                v = _gather(param, m, data_of[id(param)], shape).astype(dt)
+        elif kind == "Unary":
+            a = ev(node.operand)
+            with np.errstate(all="ignore"):
+                if node.op == "neg":
+                    v = -a
+                elif node.op == "abs":
+                    v = np.abs(a)
+                else:
+                    fn = {"sqrt": np.sqrt, "tanh": np.tanh, "exp": np.exp, "log": np.log}[node.op]
+                    v = fn(a.astype(np.float64)).astype(dt)
+        elif kind == "Binary":
+            a, b = ev(node.lhs), ev(node.rhs)
+            with np.errstate(all="ignore"):
+                v = {"+": np.add, "-": np.subtract, "*": np.multiply, "/": np.divide, "%": np.fmod,
+                     "min": np.fmin, "max": np.fmax}[node.op](a, b)
+        else:
+            raise TypeError("unexpected node " + kind)
+        v = np.asarray(v, dtype=dt)
+        memo[key] = v
+        return v
+
+    if type(closure).__name__ == "Concatenate":
+        return np.stack([ev(e) for e in closure.elements], axis=-1)
+    return ev(closure)
+
+
+def _param_order(closure):
+    seen, out, stack = set(), [], [closure]
+    while stack:
+        n = stack.pop()
+        if id(n) in seen:
+            continue
+        seen.add(id(n))
+        k = type(n).__name__
+        if k == "ArrayParameter":
+            out.append(n)
+            continue
+        kids = {"Transform": lambda: [n.array], "Extract": lambda: [n.array], "Fill": lambda: [n.element],
+                "Unary": lambda: [n.operand], "Binary": lambda: [n.lhs, n.rhs],
+                "Concatenate": lambda: list(n.elements)}.get(k, lambda: [])()
+        stack.extend(reversed(kids))
+    return out
+
+
+# ----------------------------------------------------------------------------- metrics
+def ulp_distance(a, b):
+    """|a - b| in units in the last place of FP32, elementwise (NaN vs NaN = 0)."""
+    a = np.asarray(a, dtype=np.float32).ravel()
+    b = np.asarray(b, dtype=np.float32).ravel()
+    ia = a.view(np.int32).astype(np.int64)
+    ib = b.view(np.int32).astype(np.int64)
+    oa = np.where(ia < 0, -0x80000000 - ia, ia)
+    ob = np.where(ib < 0, -0x80000000 - ib, ib)
+    d = np.abs(oa - ob)
+    both_nan = np.isnan(a) & np.isnan(b)
+    d[both_nan] = 0
+    one_nan = np.isnan(a) ^ np.isnan(b)
+    d[one_nan] = np.iinfo(np.int64).max
+    return d
+
+
+EPS32 = 2.0 ** -23
+
+
+def sum_bound(x, axis=None):
+    """The reduction tolerance 4 * eps32 * sum |x| (FP64)."""
+    return 4.0 * EPS32 * np.sum(np.abs(np.asarray(x, dtype=np.float64)), axis=axis)
+
+
+def gemm_fp64(a, b):
+    return np.asarray(a, np.float64) @ np.asarray(b, np.float64)
+
+
+def gemm_normalized_error(c, a, b, c64=None):
+    """max_ij |C - C64| / sum_k |a_ik| |b_kj|."""
+    if c64 is None:
+        c64 = gemm_fp64(a, b)
+    denom = np.abs(np.asarray(a, np.float64)) @ np.abs(np.asarray(b, np.float64))
+    denom[denom == 0] = 1.0
+    return float(np.max(np.abs(np.asarray(c, np.float64) - c64) / denom))

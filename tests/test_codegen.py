@@ -1,0 +1,140 @@
+"""HIP code generation without a device: the emitted text folds views into index math,
+compiles for gfx950 with hiprtc, and rejects malformed DAGs (CPU)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from compute import _capi as capi
+from compute import trees
+
+
+def compiles(src, opts=b"-ffp-contract=off"):
+    log = ctypes.c_void_p()
+    st = capi.lib.cs_program_check(src.encode(), opts, ctypes.byref(log))
+    text = capi.take_string(log.value)
+    assert st == 0, text
+    return True
+
+
+def c2_expr(T):
+    X = T.Tensor(np.zeros((64, 256, 256), np.float32))
+    bias = T.Tensor(np.zeros((256,), np.float32))
+    p = X.permute([1, 0, 2])
+    y = p + bias.broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+    return T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+
+
+def test_c1_kernel_text(offline_tensors):
+    T = offline_tensors
+    a, b, c = (T.Tensor(np.zeros((1024, 1024), np.float32)) for _ in range(3))
+    r = a * b + c
+    src = T.kernel_source(r.closure, r.shape)
+    assert src.count("const csmi_f4 a_") == 3  # three wide loads
+    assert "t4[v] = t1[v] * t3[v]" in src and "t7[v] = t4[v] + t6[v]" in src
+    assert "fma" not in src
+    compiles(src)
+
+
+def test_c2_views_fold_into_one_kernel(offline_tensors):
+    T = offline_tensors
+    out = c2_expr(T)
+    nodes, outputs, leaves = trees.serialize(out.closure)
+    assert len(leaves) == 2  # X once (both views share its parameter) and bias
+    src = T.kernel_source(out.closure, out.shape)
+    assert "p0 + (g2_i0 * 65536 + g2_i1 * 256 + g2_i2)" in src  # permute(1,0,2)
+    assert "(int)(c2 + -1)" in src  # translate(0,0,1)
+    assert "p1[g5_i0]" in src  # bias aligned with dimension 0, one scalar load
+    assert "expf(" in src and "tanhf(" in src
+    compiles(src)
+
+
+@pytest.mark.parametrize("shape", [(), (7,), (3, 5), (2, 3, 4, 5), (2, 1, 3, 1, 2, 2)])
+def test_shapes_compile(offline_tensors, shape):
+    T = offline_tensors
+    x = T.Tensor(np.zeros(shape, np.float32))
+    r = T.Tensor.max(x + T.Tensor.fill(1.0, shape), x * x) - T.Tensor.sqrt(T.Tensor.abs(x))
+    compiles(T.kernel_source(r.closure, r.shape))
+
+
+def test_fractional_scale_and_tuple_compile(offline_tensors):
+    T = offline_tensors
+    x = T.Tensor(np.zeros((4, 6), np.float32))
+    s = x.scale((6, 9))
+    src = T.kernel_source(s.closure, s.shape)
+    assert "0.667" in src  # coefficient printed with three decimals, as the reference does
+    compiles(src)
+    j = T.Tensor.join([x, x + x, x * x])
+    src = T.kernel_source(trees.Concatenate([p.closure for p in j._parts]), (4, 6))
+    compiles(src)
+
+
+def test_fast_math_mode_compiles(offline_tensors):
+    from compute import Tensors
+    T = Tensors(offline=True, math_mode=capi.MATH_FAST)
+    out = c2_expr(T)
+    src = T.kernel_source(out.closure, out.shape)
+    assert "__expf(" in src
+    compiles(src, b"-ffast-math -ffp-contract=fast")
+
+
+def test_infinite_literal_and_mod_compile(offline_tensors):
+    T = offline_tensors
+    x = T.Tensor(np.zeros((8,), np.float32))
+    r = T.Tensor.min(x % T.Tensor.fill(3.0, (8,)), T.Tensor.fill(float("inf"), (8,)))
+    src = T.kernel_source(r.closure, r.shape)
+    assert "fmodf(" in src and "__builtin_huge_valf()" in src
+    compiles(src)
+
+
+def _gen(nodes, outputs, shape):
+    outs = (ctypes.c_int32 * len(outputs))(*outputs)
+    shp = (ctypes.c_int32 * max(len(shape), 1))(*shape)
+    src = ctypes.c_void_p()
+    st = capi.lib.cs_fused_generate_source(nodes, len(nodes), outs, len(outputs), len(shape), shp, None,
+                                           ctypes.byref(src), None)
+    capi.take_string(src.value)
+    return st
+
+
+def test_malformed_dags_are_rejected():
+    nodes = (capi.CsNode * 2)()
+    nodes[0].op = capi.OP_PARAMETER
+    nodes[0].param = 1  # must be 0: leaves are numbered in first-visit order
+    nodes[0].rank = 1
+    nodes[0].shape[0] = 4
+    nodes[1].op = capi.OP_EXTRACT
+    nodes[1].a = 0
+    assert _gen(nodes, [1], [4]) == -30
+    nodes[0].param = 0
+    assert _gen(nodes, [1], [4]) == 0
+    assert _gen(nodes, [0], [4]) == -30  # an array is not a float output
+    nodes[1].a = 5
+    assert _gen(nodes, [1], [4]) == -30  # operand out of range
+    nodes[1].a = 1
+    assert _gen(nodes, [1], [4]) == -30  # cycle
+    nodes[1].a = 0
+    assert _gen(nodes, [1], [1 << 16, 1 << 16]) == -61  # index math is 32-bit
+    nodes[0].rank = 3
+    nodes[0].shape[1] = nodes[0].shape[2] = 4
+    assert _gen(nodes, [1], [4]) == -30  # a rank-3 leaf cannot be read in a rank-1 kernel
+
+
+def test_structural_key_ignores_leaf_identity(offline_tensors):
+    T = offline_tensors
+
+    def key(expr):
+        nodes, outputs, _ = trees.serialize(expr.closure)
+        outs = (ctypes.c_int32 * len(outputs))(*outputs)
+        shp = (ctypes.c_int32 * len(expr.shape))(*expr.shape)
+        src = ctypes.c_void_p()
+        k = ctypes.c_uint64()
+        assert capi.lib.cs_fused_generate_source(nodes, len(nodes), outs, len(outputs), len(expr.shape), shp,
+                                                 None, ctypes.byref(src), ctypes.byref(k)) == 0
+        capi.take_string(src.value)
+        return k.value
+
+    a, b = T.Tensor(np.zeros((4, 4), np.float32)), T.Tensor(np.ones((4, 4), np.float32))
+    assert key(a * b + a) == key(b * a + b)
+    assert key(a * b + a) != key(a * b + b)
+    assert key(a + b) != key(a.transpose() + b)

@@ -1,0 +1,84 @@
+"""Full-size parity of the fused kernels of the two elementwise workloads:
+  a*b+c on [1024,1024] (U[-1,1], seeds 1001-1003), and
+  the view chain permute(1,0,2) + bias broadcast + translate(0,0,1), then
+  exp(-y*y)*tanh(y), on X [64,256,256] ~ N(0,1) (seed 2001), bias ~ U(-0.5,0.5) (seed 2002).
+Tolerances: views and + - * exact (bit-exact against the oracle); exp/tanh results within
+2 ulp of the FP32 program with correctly rounded operations."""
+import numpy as np
+import pytest
+
+from oracle import fp64, inputs, refcl
+
+pytestmark = pytest.mark.gpu
+
+
+def test_c1_fused_multiply_add_bit_exact(gpu_tensors):
+    T = gpu_tensors
+    a, b, c = inputs.c1_inputs()
+    r = T.Tensor(a) * T.Tensor(b) + T.Tensor(c)
+    got = r.flatArray()
+    want = ((a * b) + c).ravel()  # IEEE FP32, no contraction
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    k = refcl.Kernel(r.closure, r.shape)
+    assert np.array_equal(k([a, b, c]).view(np.uint32), got.view(np.uint32))
+
+
+def _c2(T, X, bias):
+    p = T.Tensor(X).permute([1, 0, 2])
+    y = p + T.Tensor(bias).broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+    return y, T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+
+
+def test_c2_view_chain(gpu_tensors):
+    T = gpu_tensors
+    X, bias = inputs.c2_inputs()
+    y, out = _c2(T, X, bias)
+    y_gpu = y.flatArray().reshape(256, 64, 256)
+    # views and adds: bit exact against an independent numpy statement
+    P = np.transpose(X, (1, 0, 2))
+    shifted = np.zeros_like(P)
+    shifted[:, :, 1:] = P[:, :, :-1]
+    y_np = (P + bias[:, None, None]) + shifted
+    assert np.array_equal(y_gpu.view(np.uint32), y_np.view(np.uint32))
+    # and against the reference's kernel text run as host C
+    ky = refcl.Kernel(y.closure, y.shape)
+    assert np.array_equal(ky([X, bias]).reshape(y_np.shape).view(np.uint32), y_gpu.view(np.uint32))
+    o_gpu = out.flatArray()
+    want = fp64.evaluate(out.closure, out.shape, [X, bias], "fp32").ravel()
+    d = fp64.ulp_distance(o_gpu, want)
+    print("c2 output ulp: max %d, mean %.4f, > 1ulp: %d" % (d.max(), d.mean(), int((d > 1).sum())))
+    assert d.max() <= 2
+    ko = refcl.Kernel(out.closure, out.shape)
+    d2 = fp64.ulp_distance(o_gpu, ko([X, bias]))
+    print("c2 vs refcl-host (glibc): max %d" % d2.max())
+    assert d2.max() <= 4
+
+
+@pytest.mark.parametrize("opts", [dict(vector_width=1), dict(vector_width=2, items_per_thread=3),
+                                  dict(vector_width=8, block_size=64), dict(items_per_thread=8, block_size=1024)])
+def test_launch_shapes_agree(opts):
+    """Every vector width / items-per-thread / block size computes identical results."""
+    from compute import Tensors
+    with Tensors(0, **opts) as T:
+        X, bias = inputs.c2_inputs()
+        y, out = _c2(T, X, bias)
+        ref = Tensors(0)
+        try:
+            _, out_ref = _c2(ref, X, bias)
+            assert np.array_equal(out.flatArray().view(np.uint32), out_ref.flatArray().view(np.uint32))
+        finally:
+            ref.close()
+
+
+def test_edges_and_odd_shapes(gpu_tensors):
+    T = gpu_tensors
+    for shape in [(1,), (3,), (5, 7), (2, 3, 5, 7), (1, 1, 1, 1, 1, 3)]:
+        n = int(np.prod(shape))
+        x = inputs.normal(n, 5).reshape(shape)
+        t = T.Tensor(x)
+        off = [1] + [0] * (len(shape) - 1)
+        e = t.translate(off) * t + t
+        want = fp64.evaluate(e.closure, e.shape, [x], "fp32").ravel()
+        assert np.array_equal(e.flatArray().view(np.uint32), want.view(np.uint32)), shape
+    empty = T.Tensor(np.zeros((0, 4), np.float32))
+    assert (empty + empty).flatArray().size == 0

@@ -1,0 +1,86 @@
+"""Reductions against the FP64 oracle: sum within 4*eps32*sum|x|, max exact.
+Full size: A [16384,4096] ~ N(0,1) with 1% of entries x1e4 (seeds 3001/3002)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+import oracle
+from compute import _capi as capi
+from gpu_util import alloc, download, upload
+from oracle import fp64, inputs
+
+pytestmark = pytest.mark.gpu
+
+
+def test_sum_known_answer(gpu_tensors):
+    assert gpu_tensors.Tensor.fill(15625.0, (8, 8)).sum().toString() == "1000000.0"
+
+
+@pytest.mark.parametrize("n", [0, 1, 3, 255, 256, 1000, 4097, 1 << 20, (1 << 20) + 13])
+def test_full_sum_sizes(gpu_tensors, n):
+    T = gpu_tensors
+    x = inputs.heavy_tails(inputs.normal(n, 77), 78) if n else np.zeros(0, np.float32)
+    b = upload(T, x)
+    out = alloc(T, 1)
+    out64 = alloc(T, 1, np.float64)
+    capi.check(capi.lib.cs_reduce_sum(T.stream, b.ptr, n, out.ptr, out64.ptr))
+    s = download(T, out, 1)[0]
+    s64 = download(T, out64, 1, np.float64)[0]
+    exact = x.astype(np.float64).sum()
+    assert abs(float(s) - exact) <= fp64.sum_bound(x) + abs(exact) * 2 ** -24
+    assert abs(s64 - exact) <= 1e-12 * max(np.abs(x).sum(), 1.0)
+    if n:
+        capi.check(capi.lib.cs_reduce_max(T.stream, b.ptr, n, out.ptr))
+        assert download(T, out, 1)[0] == x.max()
+    for h in (b, out, out64):
+        h.release()
+
+
+def test_max_ignores_nan(gpu_tensors):
+    T = gpu_tensors
+    x = np.array([np.nan, -3.0, 2.0, np.nan, -7.0], np.float32)
+    t = T.Tensor(x)
+    assert t.maxValue().flatArray()[0] == 2.0
+
+
+def test_rows_full_size(gpu_tensors):
+    T = gpu_tensors
+    rows, cols = 16384, 4096
+    a = inputs.c3_input(rows, cols)
+    b = upload(T, a)
+    rs, rm, tot = alloc(T, rows), alloc(T, rows), alloc(T, 1)
+    tot64 = alloc(T, 1, np.float64)
+    for _ in range(2):  # the ticket resets itself between calls
+        capi.check(capi.lib.cs_reduce_rows(T.stream, b.ptr, rows, cols, rs.ptr, rm.ptr, tot.ptr, tot64.ptr))
+    s64, m64, sa, total = oracle.row_stats64(a)
+    got_s = download(T, rs, rows)
+    got_m = download(T, rm, rows)
+    err = np.abs(got_s.astype(np.float64) - s64)
+    bound = 4 * fp64.EPS32 * sa
+    print("row sum error / bound: max %.3g" % (err / bound).max())
+    assert np.all(err <= bound)
+    assert np.array_equal(got_m, m64)
+    t = float(download(T, tot, 1)[0])
+    t64 = float(download(T, tot64, 1, np.float64)[0])
+    assert abs(t - total) <= 4 * fp64.EPS32 * sa.sum()
+    assert abs(t64 - total) <= 1e-12 * sa.sum()
+    # the left fold the reference would build (split(1).reduce(_ + _)) sits inside the same bound
+    fold_s, _ = oracle.row_fold(a)
+    assert np.all(np.abs(fold_s.astype(np.float64) - s64) <= cols * fp64.EPS32 * sa)
+    for h in (b, rs, rm, tot, tot64):
+        h.release()
+
+
+@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 1023), (129, 4099), (1000, 3)])
+def test_rows_and_cols_ragged(gpu_tensors, shape):
+    T = gpu_tensors
+    rows, cols = shape
+    a = inputs.normal(rows * cols, 9).reshape(shape)
+    t = T.Tensor(a)
+    s64, m64, sa, _ = oracle.row_stats64(a)
+    assert np.all(np.abs(t.rowSum().flatArray() - s64) <= 4 * fp64.EPS32 * sa + 1e-30)
+    assert np.array_equal(t.rowMax().flatArray(), m64)
+    cs = t.colSum().flatArray()
+    c64 = a.astype(np.float64).sum(axis=0)
+    assert np.all(np.abs(cs - c64) <= 4 * fp64.EPS32 * np.abs(a).astype(np.float64).sum(axis=0) + 1e-30)

@@ -1,0 +1,163 @@
+"""The runtime half of the C ABI: arena determinism, staging, programs, launches,
+events, graphs and error mapping."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from compute import _capi as capi
+from gpu_util import alloc, download, upload
+
+pytestmark = pytest.mark.gpu
+lib = capi.lib
+
+
+def test_device_info(gpu_tensors):
+    T = gpu_tensors
+    assert T.info(capi.INFO_DEVICE_TYPE) == 4
+    assert T.info(capi.INFO_WAVEFRONT_SIZE) == 64
+    assert T.info(capi.INFO_COMPUTE_UNITS) >= 1
+    name = ctypes.c_char_p()
+    capi.check(lib.cs_device_name(T.device, ctypes.byref(name)))
+    assert name.value.startswith(b"gfx9")
+
+
+def test_arena_is_deterministic(gpu_tensors):
+    T = gpu_tensors
+
+    def sequence():
+        bufs = [alloc(T, n) for n in (1000, 1 << 20, 7, 1 << 16)]
+        addrs = [b.ptr for b in bufs]
+        bufs[1].release()
+        c = alloc(T, 5000)
+        addrs.append(c.ptr)
+        for b in (bufs[0], bufs[2], bufs[3], c):
+            b.release()
+        return addrs
+
+    T.synchronize()
+    a1 = sequence()
+    a2 = sequence()
+    assert a1 == a2
+    assert all(p % 256 == 0 for p in a1)
+
+
+def test_staging_round_trip_large(gpu_tensors):
+    T = gpu_tensors
+    x = np.random.default_rng(0).standard_normal(40 << 20 >> 2).astype(np.float32)  # crosses staging chunks
+    b = upload(T, x)
+    assert np.array_equal(download(T, b, x.size), x)
+    y = np.empty(1000, np.float32)
+    capi.check(lib.cs_buffer_read(T.stream, b.handle, 4 * 123, y.ctypes.data_as(ctypes.c_void_p), 4000))
+    assert np.array_equal(y, x[123:1123])
+    with pytest.raises(capi.InvalidValue):
+        capi.check(lib.cs_buffer_read(T.stream, b.handle, b.nbytes - 4, y.ctypes.data_as(ctypes.c_void_p), 8))
+    b.release()
+
+
+SRC = b"""
+extern "C" __global__ void axpy(float a, const float* x, float* y, int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i] + y[i];
+}
+"""
+
+
+def test_program_build_and_launch(gpu_tensors):
+    T = gpu_tensors
+    prog, log = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(lib.cs_program_build(T.device, SRC, b"-ffp-contract=off", ctypes.byref(prog), ctypes.byref(log)))
+    capi.take_string(log.value)
+    k = ctypes.c_void_p()
+    capi.check(lib.cs_program_get_kernel(prog, b"axpy", ctypes.byref(k)))
+    with pytest.raises(capi.InvalidKernelName):
+        capi.check(lib.cs_program_get_kernel(prog, b"nope", ctypes.byref(ctypes.c_void_p())))
+    n = 1000
+    x, y = np.arange(n, dtype=np.float32), np.ones(n, np.float32)
+    bx, by = upload(T, x), upload(T, y)
+    a = ctypes.c_float(2.0)
+    px, py, nn = ctypes.c_void_p(bx.ptr), ctypes.c_void_p(by.ptr), ctypes.c_int(n)
+    args = (ctypes.c_void_p * 4)(ctypes.addressof(a), ctypes.addressof(px), ctypes.addressof(py), ctypes.addressof(nn))
+    grid = (ctypes.c_uint32 * 3)(4, 1, 1)
+    block = (ctypes.c_uint32 * 3)(256, 1, 1)
+    capi.check(lib.cs_launch(T.stream, k, grid, block, 0, args, 4))
+    assert np.array_equal(download(T, by, n), 2 * x + 1)
+    bad = (ctypes.c_uint32 * 3)(2048, 1, 1)
+    with pytest.raises(capi.InvalidWorkGroupSize):
+        capi.check(lib.cs_launch(T.stream, k, grid, bad, 0, args, 4))
+    capi.check(lib.cs_kernel_release(k))
+    capi.check(lib.cs_program_release(prog))
+    bx.release()
+    by.release()
+
+
+def test_build_failure_carries_log(gpu_tensors):
+    prog, log = ctypes.c_void_p(), ctypes.c_void_p()
+    st = lib.cs_program_build(gpu_tensors.device, b"extern \"C\" __global__ void k() { nope(); }", None,
+                              ctypes.byref(prog), ctypes.byref(log))
+    assert st == -11
+    assert "nope" in capi.take_string(log.value)
+
+
+def test_events_and_completion_callback(gpu_tensors):
+    T = gpu_tensors
+    e1, e2 = ctypes.c_void_p(), ctypes.c_void_p()
+    capi.check(lib.cs_event_record(T.stream, ctypes.byref(e1)))
+    b = alloc(T, 1 << 20)
+    capi.check(lib.cs_random(T.stream, b.ptr, 1 << 20, 1))
+    capi.check(lib.cs_event_record(T.stream, ctypes.byref(e2)))
+    done = []
+    CB = ctypes.CFUNCTYPE(None, ctypes.c_void_p)
+    cb = CB(lambda user: done.append(1))
+    capi.check(lib.cs_event_on_complete(e2, ctypes.cast(cb, ctypes.c_void_p), None))
+    capi.check(lib.cs_event_wait(e2))
+    ms = ctypes.c_float()
+    capi.check(lib.cs_event_elapsed_ms(e1, e2, ctypes.byref(ms)))
+    assert ms.value >= 0
+    c = ctypes.c_int()
+    capi.check(lib.cs_event_query(e2, ctypes.byref(c)))
+    assert c.value == 1
+    import time
+    for _ in range(200):
+        if done:
+            break
+        time.sleep(0.01)
+    assert done
+    for e in (e1, e2):
+        capi.check(lib.cs_event_release(e))
+    b.release()
+
+
+def test_graph_replays_fused_and_reduction(gpu_tensors):
+    """A captured batch of fused launches and a ticketed reduction replays correctly
+    several times, with eager work and synchronisation between replays."""
+    T = gpu_tensors
+    n = 1 << 18
+    x = np.random.default_rng(1).standard_normal(n).astype(np.float32)
+    tx = T.Tensor(x).doCache()
+    e = T.Tensor.tanh(tx) * tx + tx
+    f, leaves, _ = T.compile(e.closure, e.shape)
+    out = alloc(T, n)
+    s = alloc(T, 1)
+    xb = tx.buffer
+    leaf_ptrs = (ctypes.c_void_p * 1)(xb.ptr)
+    capi.check(lib.cs_reduce_sum(T.stream, out.ptr, n, s.ptr, None))  # size the scratch outside capture
+    T.synchronize()
+    capi.check(lib.cs_graph_begin_capture(T.stream))
+    capi.check(lib.cs_fused_launch(T.stream, f, leaf_ptrs, 1, out.ptr))
+    capi.check(lib.cs_reduce_sum(T.stream, out.ptr, n, s.ptr, None))
+    g = ctypes.c_void_p()
+    capi.check(lib.cs_graph_end_capture(T.stream, ctypes.byref(g)))
+    eager = e.flatArray()
+    want_sum = eager.astype(np.float64).sum()
+    for _ in range(4):
+        capi.check(lib.cs_fill_uniform(T.stream, out.ptr, n, 5, -9.0, 9.0))  # clobber between replays
+        capi.check(lib.cs_graph_launch(T.stream, g))
+        T.synchronize()
+        assert np.array_equal(download(T, out, n), eager)
+        assert abs(float(download(T, s, 1)[0]) - want_sum) <= 4 * 2 ** -23 * np.abs(eager).astype(np.float64).sum()
+    capi.check(lib.cs_graph_release(g))
+    capi.check(lib.cs_fused_release(f))
+    tx.release()
+    out.release()
+    s.release()

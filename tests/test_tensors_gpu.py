@@ -1,0 +1,92 @@
+"""The reference's known answers through the gfx950 back-end, and agreement with the
+oracle (the reference's kernels run as host C) on the same graphs."""
+import numpy as np
+import pytest
+
+import spec_cases
+from oracle import cpu_tensors, fp64
+
+pytestmark = pytest.mark.gpu
+
+
+def _flat(t):
+    return t.flatArray()
+
+
+@pytest.mark.parametrize("name", spec_cases.CASES)
+def test_known_answers_on_gpu(name, gpu_tensors):
+    spec_cases.check(name, gpu_tensors, lambda t: t.toString(), _flat)
+
+
+def test_repeated_to_string(gpu_tensors):
+    t = gpu_tensors.Tensor(42.0)
+    for _ in range(1000):
+        assert t.toString() == "42.0"
+
+
+def test_kernel_cache_hit_for_equal_fill(gpu_tensors):
+    """TensorsSpec.scala:37-55: a structurally equal closure finds the cached kernel."""
+    T = gpu_tensors
+    filled = T.Tensor.fill(42.0, (2, 3, 5))
+    a = filled.flatArray()
+    assert a.size == 30 and np.all(a == 42.0)
+    assert T.kernel_cache_contains(filled.closure, filled.shape)
+    zeros2 = T.Tensor.fill(42.0, (2, 3, 5))
+    assert T.kernel_cache_contains(zeros2.closure, zeros2.shape)
+
+
+def test_ragged_input_raises(gpu_tensors):
+    from compute import IllegalArgumentException
+    with pytest.raises(IllegalArgumentException):
+        gpu_tensors.Tensor([[1.0], [3.0, 4.0]])
+
+
+def test_do_cache_lifetime(gpu_tensors):
+    """README.md:172-229: the cached buffer is reused until released."""
+    T = gpu_tensors
+    a = T.Tensor([[1.0, 2.0, 3.0], [4.0, 5.0, 6.0]])
+    b = T.Tensor([[7.0, 8.0, 9.0], [10.0, 11.0, 12.0]])
+    c = T.Tensor([[13.0, 14.0, 15.0], [16.0, 17.0, 18.0]])
+    result = a * b + c
+    launches = T.launches
+    cached = result.doCache()
+    assert T.launches == launches + 1
+    assert cached.toString() == "[[20.0,30.0,42.0],[56.0,72.0,90.0]]"
+    assert cached.toString() == "[[20.0,30.0,42.0],[56.0,72.0,90.0]]"
+    assert T.launches == launches + 1  # no recomputation
+    tmp = T.Tensor.exp(cached)
+    want = np.exp(np.array([[20, 30, 42], [56, 72, 90]], np.float64)).astype(np.float32)
+    assert fp64.ulp_distance(tmp.flatArray(), want).max() <= 2
+    cached.release()
+    from compute import _capi as capi
+    with pytest.raises(capi.InvalidMemObject):
+        cached.toString()
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_graphs_agree_with_oracle(gpu_tensors, seed):
+    """Random elementwise/view graphs with fusion barriers: bit-exact with the reference's
+    kernels (host C, no FMA) for + - * / min max and views, <= 2 ulp with exp/tanh/log."""
+    T = gpu_tensors
+    rng = np.random.default_rng(seed)
+    from oracle import inputs
+    shape = tuple(int(s) for s in rng.integers(1, 9, size=rng.integers(1, 5)))
+    n = int(np.prod(shape))
+    x = T.Tensor(inputs.normal(n, 10 + seed).reshape(shape))
+    y = T.Tensor(inputs.uniform(n, 20 + seed, 0.5, 2.0).reshape(shape))
+    perm = list(rng.permutation(len(shape)))
+    xp = x.permute(perm).permute(list(np.argsort(perm)))  # composed views
+    off = [int(v) for v in rng.integers(-2, 3, size=len(shape))]
+    xt = x.translate(off)
+    e = (xp * y + xt) / y - T.Tensor.max(xt, x)
+    if seed % 2:
+        e = T.Tensor.tanh(e) + T.Tensor.exp(-T.Tensor.abs(e))
+    if seed % 3 == 0:
+        e = e.nonInline + x  # a fusion barrier
+    got = e.flatArray()
+    want = cpu_tensors.materialize(e)[:n]
+    d = fp64.ulp_distance(got, want)
+    if seed % 2:
+        assert d.max() <= 2, d.max()
+    else:
+        assert d.max() == 0
