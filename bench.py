@@ -73,7 +73,8 @@ def capture(capi, stream, fns):
     return g
 
 
-def bench_c2(T, capi, steps, warmup, sets=16):
+def bench_c2(T, capi, steps, warmup, sets=16, tune=True):
+    from compute import autotune
     from compute import tensors as tz
     lib = capi.lib
     n = 64 * 256 * 256
@@ -89,7 +90,25 @@ def bench_c2(T, capi, steps, warmup, sets=16):
     p = X.permute([1, 0, 2])
     y = p + B.broadcast((256, 64, 256)) + p.translate([0, 0, 1])
     out = T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
-    f, leaves, _ = T.compile(out.closure, out.shape)
+    leaf_sets = [[xs[i].ptr, bs[i].ptr] for i in range(sets)]
+    out_ptrs = [o.ptr for o in outs]
+    tuning = None
+    best_cfg = None
+    if tune:
+        cands = autotune.candidates(256, vector_widths=(4, 8), items=(1, 2, 4), blocks=(256, 512),
+                                    nontemporal=(0, 1))
+        results = autotune.tune_fused(T, out.closure, out.shape, leaf_sets, out_ptrs, cands, reps=8)
+        best_cfg = results[0][0]
+        default_us = [us for cfg, us in results if cfg == {"vector_width": 4, "items_per_thread": 2,
+                                                          "block_size": 256, "nontemporal_store": 0}]
+        tuning = {"best": best_cfg, "best_us": results[0][1], "default_us": default_us[0] if default_us else None,
+                  "worst_us": results[-1][1], "candidates": len(results)}
+        log("c2 tuning: best %s %.2f us, default %s us, worst %.2f us" %
+            (best_cfg, results[0][1], tuning["default_us"], results[-1][1]))
+    if best_cfg is not None:
+        f, leaves = autotune.compile_with(T, out.closure, out.shape, best_cfg)
+    else:
+        f, leaves, _ = T.compile(out.closure, out.shape)
     assert [l.id for l in leaves] == [X, B]
     ptrs = [(ctypes.c_void_p * 2)(xs[i].ptr, bs[i].ptr) for i in range(sets)]
 
@@ -112,7 +131,7 @@ def bench_c2(T, capi, steps, warmup, sets=16):
     bytes_per = 4 * n + 4 * 256 + 4 * n
     res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
-               gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9)
+               gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
     lib.cs_graph_release(g)
     lib.cs_graph_release(g_hot)
     lib.cs_fused_release(f)
@@ -243,7 +262,7 @@ def cpu_baseline_c2(seconds=10.0):
         k([X, bias])
         runs += 1
         el = time.perf_counter() - t0
-        if el >= seconds or runs >= 200:
+        if el >= seconds:
             break
     bytes_per = 4 * X.size + 4 * 256 + 4 * X.size
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
@@ -301,7 +320,8 @@ def main():
         "config": {"workload": "view-heavy fused chain: X[64,256,256] N(0,1), bias[256] U(-0.5,0.5); "
                                "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
                                "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
-                   "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2)},
+                   "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
+                   "launch_config": c2["tuning"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
         "cpu_baseline": cpu,

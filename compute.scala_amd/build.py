@@ -17,7 +17,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libcsmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip"]
+SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip", "mathcheck.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -26,11 +26,19 @@ COMMON = [
 ]
 
 
+def _embed_headers():
+    """Writes the device headers that generated kernels include as C++ string literals."""
+    with open(os.path.join(CSRC, "precise_math.h")) as f:
+        text = f.read()
+    with open(os.path.join(OBJ, "precise_math.inc"), "w") as f:
+        f.write('static const char kPreciseMathSource[] = R"csmi_src(%s)csmi_src";\n' % text)
+
+
 def _compile(src):
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src + ".o")
     lang = ["-x", "hip"] if src.endswith(".hip") else []
-    cmd = [HIPCC] + lang + COMMON + ["-c", path, "-o", obj]
+    cmd = [HIPCC] + lang + COMMON + ["-I", OBJ, "-c", path, "-o", obj]
     proc = subprocess.run(cmd, capture_output=True, text=True)
     return src, cmd, proc
 
@@ -38,6 +46,7 @@ def _compile(src):
 def build(verbose=False):
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB_DIR, exist_ok=True)
+    _embed_headers()
     with concurrent.futures.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         results = list(ex.map(_compile, SOURCES))
     failed = False

@@ -200,6 +200,7 @@ _PROTOS = {
     "cs_fill_uniform": (I, [P, P, I64, U32, D, D]),
     "cs_fill_normal": (I, [P, P, I64, U32]),
     "cs_gemm": (I, [P, I, I, I, I, P, I, P, I, P, I]),
+    "cs_math_check": (I, [P, I, U32, U32, P]),
 }
 
 for _name, (_res, _args) in _PROTOS.items():

@@ -15,6 +15,7 @@
 #include <sstream>
 
 #include "internal.h"
+#include "precise_math.inc"  // kPreciseMathSource, generated from precise_math.h
 
 struct cs_fused {
   std::atomic<int> refs{1};
@@ -343,6 +344,7 @@ class Emitter {
       << " leaves, output [";
     for (int i = 0; i < r; ++i) s << (i ? "," : "") << dag_.out_shape[i];
     s << "] x " << nout << "\n";
+    if (o_.math_mode == CS_MATH_PRECISE) s << kPreciseMathSource << "\n";
     s << "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
     s << "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     s << "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
@@ -447,8 +449,8 @@ class Emitter {
       case CS_OP_NEG: e = "-" + a; break;
       case CS_OP_ABS: e = "fabsf(" + a + ")"; break;
       case CS_OP_SQRT: e = "sqrtf(" + a + ")"; break;
-      case CS_OP_TANH: e = "tanhf(" + a + ")"; break;
-      case CS_OP_EXP: e = (fast ? "__expf(" : "expf(") + a + ")"; break;
+      case CS_OP_TANH: e = (fast ? "tanhf(" : "csmi_tanhf(") + a + ")"; break;
+      case CS_OP_EXP: e = (fast ? "__expf(" : "csmi_expf(") + a + ")"; break;
       case CS_OP_LOG: e = (fast ? "__logf(" : "logf(") + a + ")"; break;
       case CS_OP_ADD: e = a + " + " + b; break;
       case CS_OP_SUB: e = a + " - " + b; break;

@@ -7,9 +7,10 @@
 //   * per-thread sums are FP32 with a TwoSum compensation term, combined across the wave
 //     with xor shuffles and across the 4 waves through LDS, still as (sum, compensation)
 //     pairs, so the in-block result is accurate to about one FP32 rounding;
-//   * block results are combined in FP64: every block writes an FP64 partial, and the last
-//     block to finish (a ticket counter the last block resets to zero) adds the partials
-//     in a fixed order, so results are deterministic run to run;
+//   * block results are combined in FP64: every block writes an FP64 partial and a second,
+//     one-workgroup kernel adds the partials in a fixed order, so results are deterministic
+//     run to run. (A last-block ticket would need an agent-scope release fence in every
+//     block, which writes back the XCD's L2 each time; the extra launch is far cheaper.);
 //   * max uses fmaxf, which returns the non-NaN operand (OpenCLKernelBuilder.scala:279-285).
 // The reference's parallel program loses elements when the local size is not a power of
 // two (Tensors.scala:375-387); these kernels sum every element.
@@ -91,55 +92,43 @@ __device__ __forceinline__ void block_reduce(Acc &a, float &m) {
   }
 }
 
-// FP64 sum of partials[0..n) by one block in a fixed order; result in thread 0.
-__device__ double block_sum64(const double *partials, int64_t n) {
-  __shared__ double l64[kBlock];
+// One workgroup of kFinish threads adds partials[0..n) in FP64 in a fixed order.
+constexpr int kFinish = 1024;
+__global__ __launch_bounds__(kFinish) void finish_sum_kernel(const double *__restrict__ partials, int64_t n,
+                                                             float *out, double *out64) {
+  __shared__ double l64[kFinish];
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kBlock) acc += __builtin_nontemporal_load(partials + i);
+  for (int64_t i = threadIdx.x; i < n; i += kFinish) acc += partials[i];
   l64[threadIdx.x] = acc;
   __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
+  for (int off = kFinish / 2; off > 0; off >>= 1) {
     if (threadIdx.x < off) l64[threadIdx.x] += l64[threadIdx.x + off];
     __syncthreads();
   }
-  return l64[0];
+  if (threadIdx.x == 0) {
+    if (out) *out = float(l64[0]);
+    if (out64) *out64 = l64[0];
+  }
 }
 
-__device__ float block_max_partials(const float *partials, int64_t n) {
-  __shared__ float lmx[kBlock];
+__global__ __launch_bounds__(kFinish) void finish_max_kernel(const float *__restrict__ partials, int64_t n,
+                                                             float *out) {
+  __shared__ float lm[kFinish];
   float m = __builtin_nanf("");
-  for (int64_t i = threadIdx.x; i < n; i += kBlock) m = nanmax(m, __builtin_nontemporal_load(partials + i));
-  lmx[threadIdx.x] = m;
+  for (int64_t i = threadIdx.x; i < n; i += kFinish) m = nanmax(m, partials[i]);
+  lm[threadIdx.x] = m;
   __syncthreads();
-  for (int off = kBlock / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) lmx[threadIdx.x] = nanmax(lmx[threadIdx.x], lmx[threadIdx.x + off]);
+  for (int off = kFinish / 2; off > 0; off >>= 1) {
+    if (threadIdx.x < off) lm[threadIdx.x] = nanmax(lm[threadIdx.x], lm[threadIdx.x + off]);
     __syncthreads();
   }
-  return lmx[0];
-}
-
-// True in every thread of the last block to arrive. Partial results written before the
-// call are visible to that block afterwards. The last block resets the ticket.
-__device__ bool last_block(unsigned *ticket, unsigned nblocks) {
-  __shared__ bool last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(ticket, 1u);
-    last = (prev == nblocks - 1);
-    if (last) atomicExch(ticket, 0u);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
+  if (threadIdx.x == 0) *out = lm[0];
 }
 
 // ----------------------------------------------------------------------------- full sum/max
 template <bool kMax>
 __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restrict__ in, int64_t n,
-                                                            double *partial64, float *partialmax,
-                                                            unsigned *ticket, float *out,
-                                                            double *out64) {
+                                                            double *partial64, float *partialmax) {
   Acc a;
   float m = __builtin_nanf("");
   const int64_t nvec = ((reinterpret_cast<uintptr_t>(in) & 15) == 0) ? n / 4 : 0;
@@ -177,17 +166,6 @@ __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restr
     if (kMax) partialmax[blockIdx.x] = m;
     else partial64[blockIdx.x] = double(a.s) + double(a.c);
   }
-  if (!last_block(ticket, gridDim.x)) return;
-  if (kMax) {
-    float r = block_max_partials(partialmax, gridDim.x);
-    if (threadIdx.x == 0) *out = r;
-  } else {
-    double r = block_sum64(partial64, gridDim.x);
-    if (threadIdx.x == 0) {
-      if (out) *out = float(r);
-      if (out64) *out64 = r;
-    }
-  }
 }
 
 // ----------------------------------------------------------------------------- rows
@@ -196,8 +174,7 @@ __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restr
 template <bool kVec>
 __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__restrict__ in, int64_t rows,
                                                              int64_t cols, float *row_sum, float *row_max,
-                                                             double *partial64, unsigned *ticket,
-                                                             float *total, double *total64) {
+                                                             double *partial64) {
   const int64_t row = blockIdx.x;
   const float *p = in + row * cols;
   Acc a;
@@ -239,43 +216,36 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__rest
     if (row_max) row_max[row] = m;
     if (partial64) partial64[row] = double(a.s) + double(a.c);
   }
-  if (!partial64) return;
-  if (!last_block(ticket, gridDim.x)) return;
-  double r = block_sum64(partial64, rows);
-  if (threadIdx.x == 0) {
-    if (total) *total = float(r);
-    if (total64) *total64 = r;
-  }
 }
 
 // ----------------------------------------------------------------------------- columns
-// Column sums: thread = column, blockIdx.y = a slice of rows; FP64 partials per slice; the
-// last slice block of each column group adds the slices in order.
+// Column sums: thread = column, blockIdx.y = a slice of rows; FP64 partials per slice, then
+// one thread per column adds its slices in order.
 __global__ __launch_bounds__(kBlock) void reduce_cols_kernel(const float *__restrict__ in, int64_t rows,
                                                              int64_t cols, int64_t rows_per_slice,
-                                                             double *partial64, unsigned *tickets,
-                                                             float *col_sum) {
+                                                             double *partial64) {
   const int64_t col = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (col >= cols) return;
   const int64_t r0 = int64_t(blockIdx.y) * rows_per_slice;
   const int64_t r1 = r0 + rows_per_slice < rows ? r0 + rows_per_slice : rows;
   Acc a;
-  if (col < cols)
-    for (int64_t r = r0; r < r1; ++r) a.add(__builtin_nontemporal_load(in + r * cols + col));
-  if (col < cols) partial64[int64_t(blockIdx.y) * cols + col] = double(a.s) + double(a.c);
-  if (!last_block(tickets + blockIdx.x, gridDim.y)) return;
-  if (col < cols) {
-    double s = 0.0;
-    for (unsigned y = 0; y < gridDim.y; ++y) s += __builtin_nontemporal_load(partial64 + int64_t(y) * cols + col);
-    col_sum[col] = float(s);
-  }
+  for (int64_t r = r0; r < r1; ++r) a.add(__builtin_nontemporal_load(in + r * cols + col));
+  partial64[int64_t(blockIdx.y) * cols + col] = double(a.s) + double(a.c);
+}
+
+__global__ __launch_bounds__(kBlock) void finish_cols_kernel(const double *__restrict__ partial64, int64_t cols,
+                                                             int64_t slices, float *col_sum) {
+  const int64_t col = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (col >= cols) return;
+  double s = 0.0;
+  for (int64_t y = 0; y < slices; ++y) s += partial64[y * cols + col];
+  col_sum[col] = float(s);
 }
 
 }  // namespace
 
-// Per-stream scratch: [0, 4 KiB) tickets (zeroed at allocation, reset by the kernels),
-// then FP64/FP32 partials.
+// Per-stream scratch holding the FP64 (or FP32 max) partials.
 struct ReduceScratch {
-  unsigned *tickets;
   double *partial64;
   float *partialmax;
 };
@@ -284,13 +254,11 @@ struct ReduceScratch {
 
 using namespace csmi;
 
-static ReduceScratch scratch_for(cs_stream *stream, size_t partial_doubles, size_t tickets) {
-  const size_t ticket_bytes = ((tickets * sizeof(unsigned) + 4095) / 4096) * 4096;
-  char *base = static_cast<char *>(stream_scratch(stream, ticket_bytes + partial_doubles * 8 + 4096));
+static ReduceScratch scratch_for(cs_stream *stream, size_t partial_doubles) {
+  char *base = static_cast<char *>(stream_scratch(stream, partial_doubles * 8 + 4096));
   ReduceScratch r;
-  r.tickets = reinterpret_cast<unsigned *>(base);
-  r.partial64 = reinterpret_cast<double *>(base + ticket_bytes);
-  r.partialmax = reinterpret_cast<float *>(r.partial64);
+  r.partial64 = reinterpret_cast<double *>(base);
+  r.partialmax = reinterpret_cast<float *>(base);
   return r;
 }
 
@@ -313,9 +281,11 @@ cs_status cs_reduce_sum(cs_stream *stream, const float *in, int64_t n, float *ou
     if (!out && !out64) fail(CS_INVALID_VALUE, "no output");
     set_device(stream->device);
     int grid = reduce_grid(stream, n);
-    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid), 1);
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid));
     hipLaunchKernelGGL(reduce_all_kernel<false>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
-                       s.partial64, s.partialmax, s.tickets, out, out64);
+                       s.partial64, s.partialmax);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partial64,
+                       int64_t(grid), out, out64);
     CS_HIP(hipGetLastError());
   });
 }
@@ -328,9 +298,11 @@ cs_status cs_reduce_max(cs_stream *stream, const float *in, int64_t n, float *ou
     if (!out) fail(CS_INVALID_VALUE, "output is null");
     set_device(stream->device);
     int grid = reduce_grid(stream, n);
-    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid), 1);
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid));
     hipLaunchKernelGGL(reduce_all_kernel<true>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
-                       s.partial64, s.partialmax, s.tickets, out, nullptr);
+                       s.partial64, s.partialmax);
+    hipLaunchKernelGGL(finish_max_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partialmax,
+                       int64_t(grid), out);
     CS_HIP(hipGetLastError());
   });
 }
@@ -345,11 +317,14 @@ cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64
     if (rows == 0) return;
     set_device(stream->device);
     bool want_total = total || total64;
-    ReduceScratch s = scratch_for(stream, want_total ? static_cast<size_t>(rows) : 0, 1);
+    ReduceScratch s = scratch_for(stream, want_total ? static_cast<size_t>(rows) : 0);
     bool vec = cols % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
     auto k = vec ? reduce_rows_kernel<true> : reduce_rows_kernel<false>;
     hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(rows)), dim3(kBlock), 0, stream->stream, in, rows, cols,
-                       row_sum, row_max, want_total ? s.partial64 : nullptr, s.tickets, total, total64);
+                       row_sum, row_max, want_total ? s.partial64 : nullptr);
+    if (want_total)
+      hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partial64, rows, total,
+                         total64);
     CS_HIP(hipGetLastError());
   });
 }
@@ -372,9 +347,11 @@ cs_status cs_reduce_cols(cs_stream *stream, const float *in, int64_t rows, int64
     int64_t per = (rows + slices - 1) / slices;
     if (per < 1) per = 1;
     slices = rows > 0 ? (rows + per - 1) / per : 1;
-    ReduceScratch s = scratch_for(stream, static_cast<size_t>(slices * cols), static_cast<size_t>(groups));
+    ReduceScratch s = scratch_for(stream, static_cast<size_t>(slices * cols));
     hipLaunchKernelGGL(reduce_cols_kernel, dim3(static_cast<unsigned>(groups), static_cast<unsigned>(slices)),
-                       dim3(kBlock), 0, stream->stream, in, rows, cols, per, s.partial64, s.tickets, col_sum);
+                       dim3(kBlock), 0, stream->stream, in, rows, cols, per, s.partial64);
+    hipLaunchKernelGGL(finish_cols_kernel, dim3(static_cast<unsigned>(groups)), dim3(kBlock), 0, stream->stream,
+                       s.partial64, cols, slices, col_sum);
     CS_HIP(hipGetLastError());
   });
 }

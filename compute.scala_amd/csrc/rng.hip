@@ -40,7 +40,9 @@ __global__ __launch_bounds__(kBlock) void random_kernel(float *out, int64_t n, u
   out[i] = static_cast<float>(h) / 4294967296.0f;
 }
 
-// Pair i writes buffer[2i], buffer[2i+1] (Tensors.scala:409-425), all in FP32.
+// Pair i writes buffer[2i], buffer[2i+1] (Tensors.scala:409-425). Every FP32 operation is
+// correctly rounded: log, cos and sin are evaluated in FP64 and rounded once, as a
+// correctly rounded libm does on the host.
 __global__ __launch_bounds__(kBlock) void random_normal_kernel(float *out, int64_t pairs, uint32_t seed) {
   int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= pairs) return;
@@ -48,10 +50,10 @@ __global__ __launch_bounds__(kBlock) void random_normal_kernel(float *out, int64
   uint32_t r2 = xorshift32(r1);
   float u1 = static_cast<float>(r1) / 4294967296.0f;
   float u2 = static_cast<float>(r2) / 4294967296.0f;
-  float r = sqrtf(-2.0f * logf(u1));
+  float r = sqrtf(-2.0f * static_cast<float>(log(static_cast<double>(u1))));
   float theta = 2.0f * 3.14159274101257324219f * u2;  // 2 * M_PI_F * u2, left to right
-  out[2 * i] = r * cosf(theta);
-  out[2 * i + 1] = r * sinf(theta);
+  out[2 * i] = r * static_cast<float>(cos(static_cast<double>(theta)));
+  out[2 * i + 1] = r * static_cast<float>(sin(static_cast<double>(theta)));
 }
 
 // Synthetic inputs: u = hash(i ^ seed) / 2^32 exactly in FP64; x = lo + (hi - lo) * u,

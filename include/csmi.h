@@ -290,6 +290,16 @@ typedef enum cs_gemm_variant {
 cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K,
                   const float *A, int lda, const float *B, int ldb, float *C, int ldc);
 
+/* ---------------------------------------------------------------- accuracy check
+ * For every FP32 bit pattern in [start, start + count), compares the elementwise function
+ * `op` (CS_OP_EXP, CS_OP_TANH, CS_OP_LOG or CS_OP_SQRT, as the precise math mode computes
+ * it) with its FP64 counterpart rounded to FP32. stats_device points at 3 zero-initialised
+ * device uint64 values that receive max(ulp distance), the count of inputs over 1 ulp and
+ * the bits of the worst input. The reference has no counterpart: its OpenCL runtime's
+ * math library is not testable from its API. */
+cs_status cs_math_check(cs_stream *stream, int op, uint32_t start, uint32_t count,
+                        unsigned long long *stats_device);
+
 #ifdef __cplusplus
 }
 #endif

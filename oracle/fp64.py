@@ -149,6 +149,102 @@ def evaluate(closure, shape, leaves, mode="fp64"):
     return ev(closure)
 
 
+def _ulp32(v):
+    """ulp of FP32 values (as FP64), ulp(0) = smallest subnormal."""
+    v = np.abs(np.asarray(v, dtype=np.float32))
+    nxt = np.nextafter(v, np.float32(np.inf))
+    return (nxt.astype(np.float64) - v.astype(np.float64))
+
+
+def evaluate_with_bound(closure, shape, leaves, ulps=2.0):
+    """(value, bound): `value` is the FP32 program with correctly rounded operations
+    (mode "fp32" of `evaluate`); `bound` is a first-order bound on how far an
+    implementation may land from it when + - * / min max are correctly rounded and each of
+    exp, tanh, log and sqrt is within `ulps` ulp of the exact result of its own inputs.
+    Errors propagate through each operation by its derivative (mean value bounds)."""
+    shape = tuple(int(s) for s in shape)
+    params = _param_order(closure)
+    data_of = {id(p): (leaves[i] if isinstance(leaves, (list, tuple)) else leaves[id(p)])
+               for i, p in enumerate(params)}
+    memo = {}
+
+    def ev(node):
+        key = id(node)
+        if key in memo:
+            return memo[key]
+        kind = type(node).__name__
+        zero = np.zeros(shape)
+        if kind == "FloatLiteral":
+            out = (np.full(shape, node.value, dtype=np.float32), zero)
+        elif kind == "Extract":
+            res = _resolve(node.array)
+            if res[0] == "fill":
+                out = ev(res[1])
+            else:
+                _, param, m = res
+                out = (_gather(param, m, data_of[id(param)], shape), zero)
+        elif kind == "Unary":
+            a, ba = ev(node.operand)
+            a64 = a.astype(np.float64)
+            with np.errstate(all="ignore"):
+                if node.op == "neg":
+                    out = (-a, ba)
+                elif node.op == "abs":
+                    out = (np.abs(a), ba)
+                else:
+                    v = {"sqrt": np.sqrt, "tanh": np.tanh, "exp": np.exp, "log": np.log}[node.op](a64).astype(np.float32)
+                    if node.op == "exp":
+                        deriv = np.exp(a64 + ba)
+                    elif node.op == "tanh":
+                        deriv = np.ones(shape)
+                    elif node.op == "log":
+                        deriv = 1.0 / np.maximum(np.abs(a64) - ba, 1e-300)
+                    else:
+                        deriv = 0.5 / np.sqrt(np.maximum(np.abs(a64) - ba, 1e-300))
+                    out = (v, deriv * ba + (ulps + 0.5) * _ulp32(v))
+        elif kind == "Binary":
+            a, ba = ev(node.lhs)
+            b, bb = ev(node.rhs)
+            a64, b64 = a.astype(np.float64), b.astype(np.float64)
+            with np.errstate(all="ignore"):
+                v = {"+": np.add, "-": np.subtract, "*": np.multiply, "/": np.divide, "%": np.fmod,
+                     "min": np.fmin, "max": np.fmax}[node.op](a, b)
+                if node.op in ("+", "-"):
+                    prop = ba + bb
+                elif node.op == "*":
+                    prop = np.abs(a64) * bb + np.abs(b64) * ba + ba * bb
+                elif node.op == "/":
+                    prop = (ba + np.abs(a64 / b64) * bb) / np.maximum(np.abs(b64) - bb, 1e-300)
+                elif node.op in ("min", "max"):
+                    prop = np.maximum(ba, bb)
+                else:
+                    prop = np.where((ba > 0) | (bb > 0), np.inf, 0.0)
+                rounding = np.where(prop > 0, _ulp32(np.abs(v.astype(np.float64)) + prop), 0.0)
+                if node.op in ("min", "max"):
+                    rounding = 0.0
+                out = (v, prop + rounding)
+        else:
+            raise TypeError("unexpected node " + kind)
+        memo[key] = out
+        return out
+
+    if type(closure).__name__ == "Concatenate":
+        vals = [ev(e) for e in closure.elements]
+        return np.stack([v for v, _ in vals], axis=-1), np.stack([b for _, b in vals], axis=-1)
+    return ev(closure)
+
+
+def within_bound(got, value, bound):
+    """True where `got` is within `bound` of `value` (NaN matches NaN, inf matches inf)."""
+    got = np.asarray(got, dtype=np.float32).ravel()
+    value = np.asarray(value, dtype=np.float32).ravel()
+    bound = np.asarray(bound, dtype=np.float64).ravel()
+    with np.errstate(all="ignore"):
+        diff = np.abs(got.astype(np.float64) - value.astype(np.float64))
+    same = (got == value) | (np.isnan(got) & np.isnan(value))
+    return same | (diff <= bound)
+
+
 def _param_order(closure):
     seen, out, stack = set(), [], [closure]
     while stack:

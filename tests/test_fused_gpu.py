@@ -54,6 +54,27 @@ def test_c2_view_chain(gpu_tensors):
     assert d2.max() <= 4
 
 
+def test_math_functions_over_every_float(gpu_tensors):
+    """exp and tanh of the precise mode, logf and sqrtf: every FP32 input (2^32 of them)
+    against FP64 rounded to FP32, on the device."""
+    import ctypes
+    from compute import _capi as capi
+    from gpu_util import alloc, download
+    T = gpu_tensors
+    stats = alloc(T, 3, np.uint64)
+    limits = {capi.OP_EXP: 1, capi.OP_TANH: 1, capi.OP_LOG: 1, capi.OP_SQRT: 0}
+    for op, limit in limits.items():
+        zero = np.zeros(3, np.uint64)
+        capi.check(capi.lib.cs_memcpy_htod(T.stream, ctypes.c_void_p(stats.ptr), zero.ctypes.data_as(ctypes.c_void_p), 24))
+        for half in range(2):
+            capi.check(capi.lib.cs_math_check(T.stream, op, half << 31, 1 << 31, stats.ptr))
+        s = download(T, stats, 3, np.uint64)
+        x = np.array([s[2]], np.uint32).view(np.float32)[0]
+        print("op %d: max ulp %d, inputs over 1 ulp %d, worst input %r" % (op, s[0], s[1], x))
+        assert s[0] <= limit, (op, s[0], x)
+    stats.release()
+
+
 @pytest.mark.parametrize("opts", [dict(vector_width=1), dict(vector_width=2, items_per_thread=3),
                                   dict(vector_width=8, block_size=64), dict(items_per_thread=8, block_size=1024)])
 def test_launch_shapes_agree(opts):

@@ -150,3 +150,30 @@ def test_row_fold_and_fp64_stats():
     assert np.array_equal(m, a.max(axis=1))
     assert np.all(np.abs(s - s64) <= 100 * fp64.EPS32 * sa)
     assert abs(total - a.astype(np.float64).sum()) <= 1e-9 * np.abs(a).sum()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_error_bound_holds_for_host_libm_and_catches_planted_errors(seed, offline_tensors):
+    """The propagated bound admits the host libm's results (each op within 1 ulp) and
+    rejects results with a planted 6-ulp error in one transcendental."""
+    T = offline_tensors
+    rng = np.random.default_rng(seed)
+    x = inputs.normal(6 * 7, 300 + seed).reshape(6, 7)
+    y = inputs.uniform(6 * 7, 400 + seed, 0.5, 2.0).reshape(6, 7)
+    tx, ty = T.Tensor(x), T.Tensor(y)
+    e = T.Tensor.tanh(tx * ty) + T.Tensor.exp(-T.Tensor.abs(tx)) - T.Tensor.log(ty) * T.Tensor.sqrt(ty)
+    k = refcl.Kernel(e.closure, e.shape)
+    got = k([x, y])
+    value, bound = fp64.evaluate_with_bound(e.closure, e.shape, [x, y], ulps=2.0)
+    assert fp64.within_bound(got, value, bound).all()
+    # plant: the exp term off by 6 ulp
+    ex = np.exp(-np.abs(x.astype(np.float64))).astype(np.float32)
+    bad_ex = ex
+    for _ in range(6):
+        bad_ex = np.nextafter(bad_ex, np.float32(np.inf))
+    th = np.tanh((x * y).astype(np.float64)).astype(np.float32)
+    lg = np.log(y.astype(np.float64)).astype(np.float32)
+    sq = np.sqrt(y.astype(np.float64)).astype(np.float32)
+    planted = (th + bad_ex) - lg * sq
+    ok = fp64.within_bound(planted, value, bound)
+    assert not ok.all()

@@ -85,8 +85,16 @@ def test_random_graphs_agree_with_oracle(gpu_tensors, seed):
         e = e.nonInline + x  # a fusion barrier
     got = e.flatArray()
     want = cpu_tensors.materialize(e)[:n]
-    d = fp64.ulp_distance(got, want)
     if seed % 2:
-        assert d.max() <= 2, d.max()
+        # transcendentals: each within 2 ulp of its own inputs, propagated through the graph
+        leaves = [cpu_tensors.materialize(p.id) for p in _params(e.closure)]
+        value, bound = fp64.evaluate_with_bound(e.closure, e.shape, leaves, ulps=2.0)
+        ok = fp64.within_bound(got, value, bound)
+        assert ok.all(), (got[~ok][:4], value.ravel()[~ok][:4], bound.ravel()[~ok][:4])
     else:
-        assert d.max() == 0
+        assert fp64.ulp_distance(got, want).max() == 0
+
+
+def _params(closure):
+    from compute import trees
+    return trees.parameters(closure)
