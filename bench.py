@@ -298,7 +298,10 @@ def main():
         also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16))
         log("c3: %s" % also["c3_rows_sum_max_total"])
     if "c4" in only:
-        also["c4_gemm_4096"] = bench_gemm(T, capi, [("fp32", capi.GEMM_FP32)])
+        also["c4_gemm_4096"] = bench_gemm(T, capi, [("fp32", capi.GEMM_FP32), ("bf16", capi.GEMM_BF16),
+                                                     ("bf16x3", capi.GEMM_BF16X3), ("fp8", capi.GEMM_FP8)])
+        g = also["c4_gemm_4096"]
+        g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_c2()

@@ -49,7 +49,7 @@ OP_LITERAL, OP_PARAMETER, OP_TRANSFORM, OP_FILL, OP_EXTRACT = 1, 2, 3, 4, 5
 OP_NEG, OP_ABS, OP_SQRT, OP_TANH, OP_EXP, OP_LOG = 10, 11, 12, 13, 14, 15
 OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD, OP_MIN, OP_MAX = 20, 21, 22, 23, 24, 25, 26
 
-MATH_PRECISE, MATH_FAST = 0, 1
+MATH_PRECISE, MATH_FAST, MATH_ACCURATE = 0, 1, 2
 
 GEMM_FP32, GEMM_BF16, GEMM_BF16X3, GEMM_FP8 = 0, 1, 2, 3
 
@@ -200,7 +200,7 @@ _PROTOS = {
     "cs_fill_uniform": (I, [P, P, I64, U32, D, D]),
     "cs_fill_normal": (I, [P, P, I64, U32]),
     "cs_gemm": (I, [P, I, I, I, I, P, I, P, I, P, I]),
-    "cs_math_check": (I, [P, I, U32, U32, P]),
+    "cs_math_check": (I, [P, I, I, U32, U32, P]),
 }
 
 for _name, (_res, _args) in _PROTOS.items():

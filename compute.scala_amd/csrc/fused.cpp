@@ -268,7 +268,7 @@ Options resolve_options(const cs_fused_options *o, const Dag &dag) {
     r.block = o->block_size;
     r.nt = o->nontemporal_store;
   }
-  if (r.math_mode != CS_MATH_PRECISE && r.math_mode != CS_MATH_FAST)
+  if (r.math_mode != CS_MATH_PRECISE && r.math_mode != CS_MATH_FAST && r.math_mode != CS_MATH_ACCURATE)
     fail(CS_INVALID_VALUE, "unknown math mode");
   int64_t last = dag.out_shape.empty() ? 1 : dag.out_shape.back();
   int64_t total = 1;
@@ -326,13 +326,19 @@ std::string canonical_key(const Dag &dag, const Options &o) {
 }
 
 // ------------------------------------------------------------------------ code generation
+// Kernel shape: each thread owns U vectors of V consecutive innermost outputs (vector u of
+// thread t in block b is q = (b*U + u)*BLOCK + t, so a wave's accesses are contiguous).
+// The body is emitted in phases -- coordinates for every u, every leaf gather for every u,
+// then the arithmetic, then the stores -- so all of a thread's loads are in flight before
+// its first use. A value is held as t<id>[U][V], or t<id>[U] when it is the same for the V
+// elements of a vector (broadcast leaves, fills, and arithmetic on those).
 class Emitter {
  public:
   Emitter(const Dag &dag, const Options &o) : dag_(dag), o_(o) {}
 
   std::string generate(int64_t *vectors_out) {
     const int r = static_cast<int>(dag_.out_shape.size());
-    const int V = o_.V;
+    const int V = o_.V, U = o_.U;
     int64_t total = 1;
     for (auto s : dag_.out_shape) total *= s;
     const int64_t vectors = total / V;
@@ -343,44 +349,49 @@ class Emitter {
     s << "// Generated by csmi for gfx950: " << dag_.nodes.size() << " nodes, " << dag_.nleaves
       << " leaves, output [";
     for (int i = 0; i < r; ++i) s << (i ? "," : "") << dag_.out_shape[i];
-    s << "] x " << nout << "\n";
-    if (o_.math_mode == CS_MATH_PRECISE) s << kPreciseMathSource << "\n";
+    s << "] x " << nout << ", V=" << V << " U=" << U << " block=" << o_.block << "\n";
+    if (o_.math_mode == CS_MATH_ACCURATE) s << kPreciseMathSource << "\n";
     s << "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
     s << "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     s << "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
     s << "extern \"C\" __global__ __launch_bounds__(" << o_.block << ") void jit_kernel(";
     for (int i = 0; i < dag_.nleaves; ++i) s << "const float* __restrict__ p" << i << ", ";
     s << "float* __restrict__ out) {\n";
-    s << "  const unsigned vbase = blockIdx.x * " << (int64_t(o_.block) * o_.U) << "u + threadIdx.x;\n";
-    s << "#pragma unroll\n";
-    s << "  for (int u = 0; u < " << o_.U << "; ++u) {\n";
-    s << "    const unsigned q = vbase + u * " << o_.block << "u;\n";
-    s << "    if (q >= " << vectors << "u) break;\n";
-    s << "    const unsigned e0 = q * " << V << "u;\n";
-    // Decode the coordinates of the first of the V outputs (row-major, constant shape).
-    if (r > 0) {
-      s << "    unsigned rest = e0;\n";
-      for (int d = r - 1; d >= 0; --d) {
-        if (d > 0) {
-          s << "    const int c" << d << " = (int)(rest % " << dag_.out_shape[d] << "u); rest /= "
-            << dag_.out_shape[d] << "u;\n";
-        } else {
-          s << "    const int c0 = (int)rest;\n";
-        }
-      }
+    s << "  const unsigned vbase = blockIdx.x * " << (int64_t(o_.block) * U) << "u + threadIdx.x;\n";
+    // phase 0: vector index, validity and coordinates of element 0 of every vector
+    s << "  unsigned q[" << U << "]; bool live[" << U << "];\n";
+    for (int d = 0; d < r; ++d) s << "  int c" << d << "[" << U << "];\n";
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
+    s << "    q[u] = vbase + u * " << o_.block << "u;\n";
+    s << "    live[u] = q[u] < " << vectors << "u;\n";
+    s << "    unsigned rest = q[u] * " << V << "u;\n";
+    for (int d = r - 1; d >= 0; --d) {
+      if (d > 0)
+        s << "    c" << d << "[u] = (int)(rest % " << dag_.out_shape[d] << "u); rest /= " << dag_.out_shape[d] << "u;\n";
+      else
+        s << "    c0[u] = (int)rest;\n";
     }
+    s << "  }\n";
     body_ = &s;
-    for (size_t i = 0; i < dag_.nodes.size(); ++i) emit_node(static_cast<int>(i));
-    // Stores: out[(e0+v)*nout + j] is one contiguous run of V*nout floats.
+    // phase 1: gathers
+    for (size_t i = 0; i < dag_.nodes.size(); ++i)
+      if (dag_.nodes[i].op == CS_OP_EXTRACT) emit_gather(static_cast<int>(i));
+    // phase 2: arithmetic
+    for (size_t i = 0; i < dag_.nodes.size(); ++i) {
+      int op = dag_.nodes[i].op;
+      if (op != CS_OP_EXTRACT && !is_array(op) && op != CS_OP_LITERAL) emit_arith(static_cast<int>(i));
+    }
+    // phase 3: stores -- out[(e0 + v) * nout + j] is one contiguous run of V * nout floats
     const int run = V * nout;
-    s << "    float* __restrict__ dst = out + (unsigned long long)e0 * " << nout << "u;\n";
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
+    s << "    if (!live[u]) continue;\n";
+    s << "    float* __restrict__ dst = out + (unsigned long long)q[u] * " << run << "u;\n";
     std::vector<std::string> vals;
     for (int v = 0; v < V; ++v)
       for (int j = 0; j < nout; ++j) vals.push_back(value_of(dag_.outputs[j], v));
     int i = 0;
-    const bool aligned16 = (run % 4 == 0);
     while (i < run) {
-      if (aligned16 && run - i >= 4) {
+      if (run % 4 == 0 && run - i >= 4) {
         s << "    " << store("dst + " + std::to_string(i),
                              "csmi_v4{" + vals[i] + ", " + vals[i + 1] + ", " + vals[i + 2] + ", " + vals[i + 3] + "}",
                              "csmi_v4")
@@ -401,63 +412,66 @@ class Emitter {
     return "*(" + type + "*)(" + addr + ") = " + val + ";";
   }
 
-  // Name of the FP32 value of float node `id` for element v.
-  std::string value_of(int id, int v) {
-    const cs_node &n = dag_.nodes[id];
-    if (n.op == CS_OP_LITERAL) return float_literal(n.value);
-    if (uniform_.count(id)) return "t" + std::to_string(id);
-    return "t" + std::to_string(id) + "[" + std::to_string(v) + "]";
+  // extract(fill(x)) stands for x
+  int resolve(int id) const {
+    auto it = fill_of_.find(id);
+    return it == fill_of_.end() ? id : resolve(it->second);
   }
 
+  // FP32 value of float node `id` for vector u (the loop variable) and element v.
+  std::string value_of(int id, int v) {
+    id = resolve(id);
+    const cs_node &n = dag_.nodes[id];
+    if (n.op == CS_OP_LITERAL) return float_literal(n.value);
+    if (uniform_.count(id)) return "t" + std::to_string(id) + "[u]";
+    return "t" + std::to_string(id) + "[u][" + std::to_string(v) + "]";
+  }
+  std::string value_in_loop(int id) {
+    id = resolve(id);
+    const cs_node &n = dag_.nodes[id];
+    if (n.op == CS_OP_LITERAL) return float_literal(n.value);
+    if (uniform_.count(id)) return "t" + std::to_string(id) + "[u]";
+    return "t" + std::to_string(id) + "[u][v]";
+  }
   bool is_uniform(int id) const {
+    id = resolve(id);
     const cs_node &n = dag_.nodes[id];
     return n.op == CS_OP_LITERAL || uniform_.count(id) > 0;
   }
 
-  void emit_node(int id) {
+  // One SSA statement per operation and element (OpenCLKernelBuilder.scala:516-571,
+  // 271-325). A value that is the same for all V elements is computed once per vector.
+  void emit_arith(int id) {
     const cs_node &n = dag_.nodes[id];
     std::ostringstream &s = *body_;
-    const int V = o_.V;
-    if (is_array(n.op) || n.op == CS_OP_LITERAL) return;
-    if (n.op == CS_OP_EXTRACT) {
-      emit_gather(id);
-      return;
-    }
-    // Arithmetic: one SSA statement per operation and element (OpenCLKernelBuilder.scala:516-571,
-    // 271-325). A value that is the same for all V elements is computed once.
+    const int V = o_.V, U = o_.U;
     bool uni = is_uniform(n.a) && (!is_binary(n.op) || is_uniform(n.b));
-    int count = uni ? 1 : V;
     std::string name = "t" + std::to_string(id);
     if (uni) {
       uniform_.insert(id);
-      s << "    float " << name << ";\n";
+      s << "  float " << name << "[" << U << "];\n";
     } else {
-      s << "    float " << name << "[" << V << "];\n";
+      s << "  float " << name << "[" << U << "][" << V << "];\n";
     }
-    s << "#pragma unroll\n    for (int v = 0; v < " << count << "; ++v) ";
-    auto arg = [&](int child) {
-      const cs_node &c = dag_.nodes[child];
-      if (c.op == CS_OP_LITERAL) return float_literal(c.value);
-      if (uniform_.count(child)) return "t" + std::to_string(child);
-      return "t" + std::to_string(child) + "[v]";
-    };
-    std::string lhs = uni ? name : name + "[v]";
-    std::string a = arg(n.a), b = is_binary(n.op) ? arg(n.b) : "";
-    bool fast = o_.math_mode == CS_MATH_FAST;
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u)\n";
+    s << "#pragma unroll\n  for (int v = 0; v < " << (uni ? 1 : V) << "; ++v) ";
+    std::string lhs = uni ? name + "[u]" : name + "[u][v]";
+    std::string a = value_in_loop(n.a), b = is_binary(n.op) ? value_in_loop(n.b) : "";
+    const bool fast = o_.math_mode == CS_MATH_FAST, acc = o_.math_mode == CS_MATH_ACCURATE;
     std::string e;
     switch (n.op) {
       case CS_OP_NEG: e = "-" + a; break;
       case CS_OP_ABS: e = "fabsf(" + a + ")"; break;
       case CS_OP_SQRT: e = "sqrtf(" + a + ")"; break;
-      case CS_OP_TANH: e = (fast ? "tanhf(" : "csmi_tanhf(") + a + ")"; break;
-      case CS_OP_EXP: e = (fast ? "__expf(" : "csmi_expf(") + a + ")"; break;
+      case CS_OP_TANH: e = (acc ? "csmi_tanhf(" : "tanhf(") + a + ")"; break;
+      case CS_OP_EXP: e = (fast ? "__expf(" : acc ? "csmi_expf(" : "expf(") + a + ")"; break;
       case CS_OP_LOG: e = (fast ? "__logf(" : "logf(") + a + ")"; break;
       case CS_OP_ADD: e = a + " + " + b; break;
       case CS_OP_SUB: e = a + " - " + b; break;
       case CS_OP_MUL: e = a + " * " + b; break;
       case CS_OP_DIV: e = a + " / " + b; break;
-      // OpenCL C has no float `%` (the reference's text fails to build, Appendix quirk);
-      // the C fmod is the meaning the operator has everywhere else.
+      // OpenCL C has no float `%` (the reference's text fails to build); the C fmod is the
+      // meaning the operator has everywhere else.
       case CS_OP_MOD: e = "fmodf(" + a + ", " + b + ")"; break;
       case CS_OP_MIN: e = "fminf(" + a + ", " + b + ")"; break;
       case CS_OP_MAX: e = "fmaxf(" + a + ", " + b + ")"; break;
@@ -470,21 +484,14 @@ class Emitter {
   // views, 435-455 for a bare parameter).
   void emit_gather(int id) {
     std::ostringstream &s = *body_;
-    const int V = o_.V;
+    const int V = o_.V, U = o_.U;
     const int r = static_cast<int>(dag_.out_shape.size());
     const cs_node &n = dag_.nodes[id];
     ArrayRef ar = resolve_array(dag_, n.a);
     std::string name = "t" + std::to_string(id);
     if (ar.is_fill) {
-      // extract(fill(x)) == x
-      const cs_node &fv = dag_.nodes[ar.fill_value];
-      if (fv.op == CS_OP_LITERAL || uniform_.count(ar.fill_value)) {
-        uniform_.insert(id);
-        s << "    const float " << name << " = " << value_of(ar.fill_value, 0) << ";\n";
-      } else {
-        s << "    float " << name << "[" << V << "];\n#pragma unroll\n    for (int v = 0; v < " << V
-          << "; ++v) " << name << "[v] = t" << ar.fill_value << "[v];\n";
-      }
+      // extract(fill(x)) == x; x is computed in the arithmetic phase, so alias it there
+      fill_of_[id] = ar.fill_value;
       return;
     }
     const cs_node &p = dag_.nodes[ar.param];
@@ -494,27 +501,26 @@ class Emitter {
     if (src_rank == 0) {
       // `*x`: a scalar source is read without bounds (OpenCLKernelBuilder.scala:351-356).
       uniform_.insert(id);
-      s << "    const float " << name << " = " << ptr << "[0];\n";
+      s << "  float " << name << "[" << U << "];\n";
+      s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) " << name << "[u] = " << ptr << "[0];\n";
       return;
     }
-    // Row y of the index map: idx_y = (int)(sum_x m[y][x] * c_x + m[y][r]).
     struct Row {
       bool empty = true;        // all-zero row: index 0, no bounds check
       bool fractional = false;  // double arithmetic, truncated by (int)
-      std::string expr;         // index of element 0 (or the per-element expression)
+      std::string expr;         // index of element 0 of the vector
       int64_t inner_step = 0;   // integer coefficient of the innermost output coordinate
       bool upper_only = false;  // bare parameter: only idx < dim is checked
     };
     std::vector<Row> rows(static_cast<size_t>(src_rank));
     if (!ar.has_view) {
-      // Bare parameter: (*x)[gid_0]...[gid_{rank-1}] with upper bounds only.
       if (src_rank > r)
         fail(CS_INVALID_VALUE, "a leaf of rank " + std::to_string(src_rank) +
                                    " cannot be read directly in a rank-" + std::to_string(r) + " kernel");
       for (int y = 0; y < src_rank; ++y) {
         rows[y].empty = false;
         rows[y].upper_only = true;
-        rows[y].expr = "c" + std::to_string(y);
+        rows[y].expr = "c" + std::to_string(y) + "[u]";
         rows[y].inner_step = (y == r - 1) ? 1 : 0;
       }
     } else {
@@ -532,11 +538,9 @@ class Emitter {
           std::string lit = decimal_format(m);
           if (lit.find('.') != std::string::npos) frac = true;
           if (x < cols - 1) {
-            if (m == 1.0) terms.push_back("c" + std::to_string(x));
-            else terms.push_back("c" + std::to_string(x) + " * " + lit);
-            if (x == r - 1) {
-              if (lit.find('.') == std::string::npos) row.inner_step = std::stoll(lit);
-            }
+            std::string cx = "c" + std::to_string(x) + "[u]";
+            terms.push_back(m == 1.0 ? cx : cx + " * " + lit);
+            if (x == r - 1 && lit.find('.') == std::string::npos) row.inner_step = std::stoll(lit);
           } else {
             terms.push_back(lit);
           }
@@ -549,57 +553,38 @@ class Emitter {
         row.expr = e;
       }
     }
-    // Source row-major strides.
     std::vector<int64_t> stride(static_cast<size_t>(src_rank), 1);
     for (int y = src_rank - 2; y >= 0; --y) stride[y] = stride[y + 1] * p.shape[y + 1];
 
     bool any_frac = false;
-    for (auto &row : rows) any_frac = any_frac || row.fractional;
-    // Classify the V elements: broadcast (no row depends on the innermost coordinate),
-    // contiguous (only the innermost source row moves, by +1), or general.
-    bool inner_static = true;
     int moving = -1, nmoving = 0;
     for (int y = 0; y < src_rank; ++y) {
       if (rows[y].empty) continue;
-      if (rows[y].fractional) { inner_static = false; continue; }
-      if (rows[y].inner_step != 0) { moving = y; ++nmoving; }
+      any_frac = any_frac || rows[y].fractional;
+      if (!rows[y].fractional && rows[y].inner_step != 0) {
+        moving = y;
+        ++nmoving;
+      }
     }
     const bool broadcast = !any_frac && nmoving == 0;
-    const bool contiguous = !any_frac && nmoving == 1 && moving == src_rank - 1 &&
-                            rows[moving].inner_step == 1 && V > 1;
-    (void)inner_static;
-
+    const bool contiguous = !any_frac && nmoving == 1 && moving == src_rank - 1 && rows[moving].inner_step == 1 && V > 1;
     const std::string g = "g" + std::to_string(id);
-    // Per-row index of element 0 and the combined bounds predicate.
-    auto emit_rows = [&](const std::string &suffix, const std::string &cinner) {
+    // Per-row indices (element 0, or per element with the innermost coordinate advanced)
+    auto emit_rows = [&](const std::string &cinner) {
       std::vector<std::string> conds;
       std::string off;
       for (int y = 0; y < src_rank; ++y) {
         const Row &row = rows[y];
-        std::string ix = g + "_i" + std::to_string(y) + suffix;
+        std::string ix = g + "_i" + std::to_string(y);
         if (row.empty) {
           s << "      const int " << ix << " = 0;\n";
         } else {
           std::string e = row.expr;
-          if (!cinner.empty() && r > 0) {
-            // replace the innermost coordinate name with the per-element one
-            std::string from = "c" + std::to_string(r - 1);
-            std::string out;
-            size_t pos = 0;
-            while (pos < e.size()) {
-              if (e.compare(pos, from.size(), from) == 0 &&
-                  (pos + from.size() == e.size() || !isdigit(static_cast<unsigned char>(e[pos + from.size()])))) {
-                out += cinner;
-                pos += from.size();
-              } else {
-                out += e[pos++];
-              }
-            }
-            e = out;
-          }
+          if (!cinner.empty() && r > 0) e = replace_coord(e, "c" + std::to_string(r - 1) + "[u]", cinner);
           s << "      const int " << ix << " = (int)(" << e << ");\n";
-          if (row.upper_only) conds.push_back(ix + " < " + std::to_string(p.shape[y]));
-          else {
+          if (row.upper_only) {
+            conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+          } else {
             conds.push_back(ix + " >= 0");
             conds.push_back(ix + " < " + std::to_string(p.shape[y]));
           }
@@ -613,75 +598,84 @@ class Emitter {
     };
 
     if (broadcast || V == 1) {
-      s << "    float " << name << "_s;\n    {\n";
-      auto oc = emit_rows("", "");
-      if (oc.second.empty())
-        s << "      " << name << "_s = " << ptr << "[" << oc.first << "];\n";
-      else
-        s << "      " << name << "_s = (" << oc.second << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
-      s << "    }\n";
-      if (V == 1) {
-        s << "    const float " << name << "[1] = {" << name << "_s};\n";
-      } else {
-        uniform_.insert(id);
-        s << "    const float " << name << " = " << name << "_s;\n";
-      }
+      uniform_.insert(id);
+      s << "  float " << name << "[" << U << "];\n";
+      s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
+      auto oc = emit_rows("");
+      std::string cond = oc.second.empty() ? "live[u]" : "live[u] && " + oc.second;
+      s << "      " << name << "[u] = (" << cond << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
+      s << "  }\n";
       return;
     }
-    s << "    float " << name << "[" << V << "];\n";
+    s << "  float " << name << "[" << U << "][" << V << "];\n";
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
     if (contiguous) {
       // One wide load when all V elements are inside the source row, else per element.
-      s << "    {\n";
-      auto oc = emit_rows("", "");
+      auto oc = emit_rows("");
       std::string yi = g + "_i" + std::to_string(moving);
-      // Bounds of element 0 on the other rows, and of elements 0..V-1 on the moving row.
-      std::vector<std::string> conds;
+      std::vector<std::string> conds{"live[u]"};
       for (int y = 0; y < src_rank; ++y) {
         if (rows[y].empty || y == moving) continue;
         std::string ix = g + "_i" + std::to_string(y);
-        if (rows[y].upper_only) conds.push_back(ix + " < " + std::to_string(p.shape[y]));
-        else { conds.push_back(ix + " >= 0"); conds.push_back(ix + " < " + std::to_string(p.shape[y])); }
+        if (rows[y].upper_only) {
+          conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+        } else {
+          conds.push_back(ix + " >= 0");
+          conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+        }
       }
       std::string others;
       for (size_t c = 0; c < conds.size(); ++c) others += (c ? " && " : "") + conds[c];
-      if (others.empty()) others = "true";
       s << "      const bool " << g << "_ok = " << others << ";\n";
       s << "      const float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
-      s << "      if (" << g << "_ok && " << yi << " >= 0 && " << yi << " + " << (V - 1) << " < " << p.shape[moving] << ") {\n";
-      if (V == 8) {
-        s << "        const csmi_f4 a_ = *(const csmi_f4*)(" << g << "_p);\n";
-        s << "        const csmi_f4 b_ = *(const csmi_f4*)(" << g << "_p + 4);\n";
-        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
-        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v + 4 << "] = b_[" << v << "];\n";
-      } else if (V == 4) {
-        s << "        const csmi_f4 a_ = *(const csmi_f4*)(" << g << "_p);\n";
-        for (int v = 0; v < 4; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
-      } else {
-        s << "        const csmi_f2 a_ = *(const csmi_f2*)(" << g << "_p);\n";
-        for (int v = 0; v < 2; ++v) s << "        " << name << "[" << v << "] = a_[" << v << "];\n";
+      s << "      if (" << g << "_ok && " << yi << " >= 0 && " << yi << " + " << (V - 1) << " < " << p.shape[moving]
+        << ") {\n";
+      for (int h = 0; h < V; h += 4) {
+        if (V - h >= 4) {
+          s << "        const csmi_f4 a" << h << "_ = *(const csmi_f4*)(" << g << "_p + " << h << ");\n";
+          for (int v = 0; v < 4; ++v)
+            s << "        " << name << "[u][" << h + v << "] = a" << h << "_[" << v << "];\n";
+        } else {
+          s << "        const csmi_f2 a" << h << "_ = *(const csmi_f2*)(" << g << "_p + " << h << ");\n";
+          for (int v = 0; v < 2; ++v)
+            s << "        " << name << "[u][" << h + v << "] = a" << h << "_[" << v << "];\n";
+        }
       }
       s << "      } else {\n#pragma unroll\n        for (int v = 0; v < " << V << "; ++v) {\n";
       s << "          const int j_ = " << yi << " + v;\n";
-      s << "          " << name << "[v] = (" << g << "_ok && j_ >= 0 && j_ < " << p.shape[moving] << ") ? "
-        << g << "_p[v] : " << pad << ";\n";
-      s << "        }\n      }\n    }\n";
+      s << "          " << name << "[u][v] = (" << g << "_ok && j_ >= 0 && j_ < " << p.shape[moving] << ") ? " << g
+        << "_p[v] : " << pad << ";\n";
+      s << "        }\n      }\n  }\n";
       return;
     }
     // General gather: every element computes its own index.
     s << "#pragma unroll\n    for (int v = 0; v < " << V << "; ++v) {\n";
-    std::string cin = "(c" + std::to_string(r - 1) + " + v)";
-    auto oc = emit_rows("", cin);
-    if (oc.second.empty())
-      s << "      " << name << "[v] = " << ptr << "[" << oc.first << "];\n";
-    else
-      s << "      " << name << "[v] = (" << oc.second << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
-    s << "    }\n";
+    std::string cin = "(c" + std::to_string(r - 1) + "[u] + v)";
+    auto oc = emit_rows(cin);
+    std::string cond = oc.second.empty() ? "live[u]" : "live[u] && " + oc.second;
+    s << "      " << name << "[u][v] = (" << cond << ") ? " << ptr << "[" << oc.first << "] : " << pad << ";\n";
+    s << "    }\n  }\n";
+  }
+
+  static std::string replace_coord(const std::string &e, const std::string &from, const std::string &to) {
+    std::string out;
+    size_t pos = 0;
+    while (pos < e.size()) {
+      if (e.compare(pos, from.size(), from) == 0) {
+        out += to;
+        pos += from.size();
+      } else {
+        out += e[pos++];
+      }
+    }
+    return out;
   }
 
   const Dag &dag_;
   Options o_;
   std::ostringstream *body_ = nullptr;
   std::set<int> uniform_;
+  std::map<int, int> fill_of_;
 };
 
 std::vector<std::string> math_options(int mode) {

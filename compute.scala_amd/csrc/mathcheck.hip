@@ -20,11 +20,13 @@ __device__ __forceinline__ int64_t ordered(float f) {
   return i < 0 ? int64_t(INT32_MIN) - i : int64_t(i);
 }
 
-__device__ float eval32(int op, float x) {
+// The functions each math mode emits (fused.cpp); the fast mode's are compiled here
+// without fast-math flags, so they check the library functions the mode calls.
+__device__ float eval32(int op, int mode, float x) {
   switch (op) {
-    case CS_OP_EXP: return csmi_expf(x);
-    case CS_OP_TANH: return csmi_tanhf(x);
-    case CS_OP_LOG: return logf(x);
+    case CS_OP_EXP: return mode == CS_MATH_ACCURATE ? csmi_expf(x) : mode == CS_MATH_FAST ? __expf(x) : expf(x);
+    case CS_OP_TANH: return mode == CS_MATH_ACCURATE ? csmi_tanhf(x) : tanhf(x);
+    case CS_OP_LOG: return mode == CS_MATH_FAST ? __logf(x) : logf(x);
     case CS_OP_SQRT: return sqrtf(x);
     default: return __builtin_nanf("");
   }
@@ -41,7 +43,7 @@ __device__ double eval64(int op, double x) {
 }
 
 // stats[0] = max ulp, stats[1] = inputs over 1 ulp, stats[2] = worst input bits
-__global__ __launch_bounds__(256) void math_check_kernel(int op, uint32_t start, uint32_t count,
+__global__ __launch_bounds__(256) void math_check_kernel(int op, int mode, uint32_t start, uint32_t count,
                                                          unsigned long long *stats) {
   uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
   uint64_t stride = uint64_t(gridDim.x) * 256;
@@ -49,7 +51,7 @@ __global__ __launch_bounds__(256) void math_check_kernel(int op, uint32_t start,
   for (; i < count; i += stride) {
     const uint32_t bits = start + static_cast<uint32_t>(i);
     const float x = __int_as_float(static_cast<int32_t>(bits));
-    const float got = eval32(op, x);
+    const float got = eval32(op, mode, x);
     const float want = static_cast<float>(eval64(op, static_cast<double>(x)));
     unsigned long long d;
     if (want != want || got != got) {
@@ -76,16 +78,18 @@ __global__ __launch_bounds__(256) void math_check_kernel(int op, uint32_t start,
 
 using namespace csmi;
 
-extern "C" cs_status cs_math_check(cs_stream *stream, int op, uint32_t start, uint32_t count,
+extern "C" cs_status cs_math_check(cs_stream *stream, int op, int mode, uint32_t start, uint32_t count,
                                    unsigned long long *stats_device) {
   return guard([&] {
     if (!stream || !stats_device) fail(CS_INVALID_VALUE, "null argument");
     if (op != CS_OP_EXP && op != CS_OP_TANH && op != CS_OP_LOG && op != CS_OP_SQRT)
       fail(CS_INVALID_VALUE, "unsupported op");
+    if (mode != CS_MATH_PRECISE && mode != CS_MATH_FAST && mode != CS_MATH_ACCURATE)
+      fail(CS_INVALID_VALUE, "unknown math mode");
     if (count == 0) return;
     set_device(stream->device);
     unsigned grid = static_cast<unsigned>(std::min<uint64_t>((uint64_t(count) + 255) / 256, 8192));
-    hipLaunchKernelGGL(math_check_kernel, dim3(grid), dim3(256), 0, stream->stream, op, start, count,
+    hipLaunchKernelGGL(math_check_kernel, dim3(grid), dim3(256), 0, stream->stream, op, mode, start, count,
                        stats_device);
     CS_HIP(hipGetLastError());
   });

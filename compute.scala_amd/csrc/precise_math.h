@@ -30,20 +30,20 @@ __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, fl
   ql = s - (qh - rh);  // Fast2Sum: |rh| >= |s|
 }
 
+// Branch free: the argument is clamped to [-104, 89] (exp(89) overflows to +inf and
+// exp(-104) rounds to +0, as the unclamped values would), and NaN is selected at the end.
 __device__ __forceinline__ float csmi_expf(float x) {
-  if (x != x) return x;
-  if (x >= 89.0f) return __builtin_huge_valf();
-  if (x <= -104.0f) return 0.0f;
+  const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
   float n, qh, ql;
-  csmi_exp_reduce(x, n, qh, ql);
+  csmi_exp_reduce(xc, n, qh, ql);
   const float t = 1.0f + qh;
   const float e = (1.0f - t) + qh;  // Fast2Sum: |qh| < 1
-  return ldexpf(t + (e + ql), static_cast<int>(n));
+  const float r = ldexpf(t + (e + ql), static_cast<int>(n));
+  return x != x ? x : r;
 }
 
 __device__ __forceinline__ float csmi_tanhf(float x) {
-  if (x != x) return x;
-  const float a = fminf(fabsf(x), 9.5f);  // tanh rounds to 1 beyond 9.01
+  const float a = fminf(fabsf(x), 9.5f);  // tanh rounds to 1 beyond 9.01; NaN selected below
   float n, qh, ql;
   csmi_exp_reduce(2.0f * a, n, qh, ql);
   const int k = static_cast<int>(n);
@@ -62,5 +62,5 @@ __device__ __forceinline__ float csmi_tanhf(float x) {
   const float t0 = eh * inv;
   const float res = fmaf(-t0, dh, eh) + fmaf(-t0, dl, el);
   const float t = fmaf(res, inv, t0);
-  return copysignf(t, x);
+  return x != x ? x : copysignf(t, x);
 }

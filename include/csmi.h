@@ -196,8 +196,11 @@ typedef struct cs_node {
 } cs_node;
 
 typedef enum cs_math_mode {
-  CS_MATH_PRECISE = 0,  /* no FMA contraction, correctly rounded + - * / sqrt */
-  CS_MATH_FAST = 1      /* contraction and fast math, as -cl-unsafe-math-optimizations */
+  CS_MATH_PRECISE = 0,  /* no FMA contraction, correctly rounded + - * / sqrt; exp, tanh and
+                           log from the device math library (each within 2 ulp) */
+  CS_MATH_FAST = 1,     /* contraction and fast math, as -cl-unsafe-math-optimizations */
+  CS_MATH_ACCURATE = 2  /* as PRECISE, with exp and tanh within 1 ulp (compensated
+                           polynomial evaluation; about twice the instructions) */
 } cs_math_mode;
 
 typedef struct cs_fused_options {
@@ -292,12 +295,12 @@ cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K,
 
 /* ---------------------------------------------------------------- accuracy check
  * For every FP32 bit pattern in [start, start + count), compares the elementwise function
- * `op` (CS_OP_EXP, CS_OP_TANH, CS_OP_LOG or CS_OP_SQRT, as the precise math mode computes
- * it) with its FP64 counterpart rounded to FP32. stats_device points at 3 zero-initialised
+ * `op` (CS_OP_EXP, CS_OP_TANH, CS_OP_LOG or CS_OP_SQRT, as math mode `mode` computes it in
+ * generated kernels) with its FP64 counterpart rounded to FP32. stats_device points at 3 zero-initialised
  * device uint64 values that receive max(ulp distance), the count of inputs over 1 ulp and
  * the bits of the worst input. The reference has no counterpart: its OpenCL runtime's
  * math library is not testable from its API. */
-cs_status cs_math_check(cs_stream *stream, int op, uint32_t start, uint32_t count,
+cs_status cs_math_check(cs_stream *stream, int op, int mode, uint32_t start, uint32_t count,
                         unsigned long long *stats_device);
 
 #ifdef __cplusplus

@@ -30,8 +30,8 @@ def test_c1_kernel_text(offline_tensors):
     a, b, c = (T.Tensor(np.zeros((1024, 1024), np.float32)) for _ in range(3))
     r = a * b + c
     src = T.kernel_source(r.closure, r.shape)
-    assert src.count("const csmi_f4 a_") == 3  # three wide loads
-    assert "t4[v] = t1[v] * t3[v]" in src and "t7[v] = t4[v] + t6[v]" in src
+    assert src.count("const csmi_f4 a0_") == 3  # three wide loads
+    assert "t4[u][v] = t1[u][v] * t3[u][v]" in src and "t7[u][v] = t4[u][v] + t6[u][v]" in src
     body = src[src.index("void jit_kernel"):]
     assert "fma" not in body  # no contraction in the precise mode
     compiles(src)
@@ -44,9 +44,14 @@ def test_c2_views_fold_into_one_kernel(offline_tensors):
     assert len(leaves) == 2  # X once (both views share its parameter) and bias
     src = T.kernel_source(out.closure, out.shape)
     assert "p0 + (g2_i0 * 65536 + g2_i1 * 256 + g2_i2)" in src  # permute(1,0,2)
-    assert "(int)(c2 + -1)" in src  # translate(0,0,1)
+    assert "(int)(c2[u] + -1)" in src  # translate(0,0,1)
     assert "p1[g5_i0]" in src  # bias aligned with dimension 0, one scalar load
     assert "expf(" in src and "tanhf(" in src
+    compiles(src)
+    from compute import Tensors
+    acc = Tensors(offline=True, math_mode=capi.MATH_ACCURATE)
+    src = acc.kernel_source(c2_expr(acc).closure, (256, 64, 256))
+    assert "csmi_expf(" in src and "csmi_tanhf(" in src
     compiles(src)
 
 

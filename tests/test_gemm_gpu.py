@@ -9,18 +9,23 @@ from oracle import fp64, inputs
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = {"fp32": capi.GEMM_FP32}
-BOUND = {"fp32": 1e-6}
+VARIANTS = {"fp32": capi.GEMM_FP32, "bf16": capi.GEMM_BF16, "bf16x3": capi.GEMM_BF16X3, "fp8": capi.GEMM_FP8}
+# Normalized error max |C - C64| / (|A| |B|) bounds: FP32 accumulation; BF16 rounds each
+# operand to 8 bits (2 * 2^-9 per product); BF16x3 keeps about 16 bits (<= 1e-5 by design);
+# e4m3 keeps 4 bits (2 * 2^-4 per product) with per-block power-of-two scales.
+BOUND = {"fp32": 1e-6, "bf16": 2 * 2 ** -9 + 1e-6, "bf16x3": 1e-5, "fp8": 2 * 2 ** -4 + 1e-3}
 
 
-def test_small_known_answer(gpu_tensors):
+@pytest.mark.parametrize("variant", ["fp32", "bf16", "bf16x3"])
+def test_small_known_answer(gpu_tensors, variant):
     T = gpu_tensors
-    c = T.Tensor.matrixMultiply(T.Tensor(spec_cases.M1), T.Tensor(spec_cases.M2))
+    c = T.Tensor.matrixMultiply(T.Tensor(spec_cases.M1), T.Tensor(spec_cases.M2), VARIANTS[variant])
     assert c.toString() == spec_cases.GOLDEN["matrix_multiplication"]["expect"]
 
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
-@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 65, 17), (128, 256, 512), (300, 200, 1000)])
+@pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 65, 17), (128, 256, 512), (300, 200, 1000), (256, 512, 384),
+                                 (513, 257, 129)])
 def test_gemm_shapes(gpu_tensors, variant, mnk):
     T = gpu_tensors
     M, N, K = mnk
@@ -52,3 +57,19 @@ def test_gemm_4096(gpu_tensors, variant):
         assert err <= BOUND[variant]
         for h in (ba, bb, bc):
             h.release()
+
+
+def test_fp8_representation(gpu_tensors):
+    """OCP e4m3 (max 448, 3 mantissa bits): a matrix whose entries are exact e4m3 values
+    times one power of two per row multiplies exactly."""
+    T = gpu_tensors
+    rng = np.random.default_rng(3)
+    vals = np.array([0.0, 0.5, 1.0, 1.5, 2.0, 3.5, 6.0, 14.0, 448.0, -1.0, -3.0], np.float32)
+    a = rng.choice(vals, size=(64, 128)).astype(np.float32)
+    b = rng.choice(vals[:6], size=(128, 64)).astype(np.float32)  # all sums stay exact in FP32
+    ba, bb, bc = upload(T, a), upload(T, b), alloc(T, 64 * 64)
+    capi.check(capi.lib.cs_gemm(T.stream, capi.GEMM_FP8, 64, 64, 128, ba.ptr, 128, bb.ptr, 64, bc.ptr, 64))
+    c = download(T, bc, 64 * 64).reshape(64, 64)
+    assert np.array_equal(c, (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32))
+    for h in (ba, bb, bc):
+        h.release()
