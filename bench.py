@@ -245,6 +245,127 @@ def bench_gemm(T, capi, variants, reps=10):
     return out
 
 
+def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001):
+    """512 uneven expressions: per-expression launches vs one persistent batch (both
+    graph-captured), with configurations from the cost model when its parameters exist.
+    With `sweep`, first runs the exhaustive autotuner over every configuration, trains the
+    cost model on 3/4 of the expressions and measures its regret on the rest."""
+    import concurrent.futures
+    from compute import batch as bm
+    from compute import costmodel as cm
+    from compute import workloads
+    t0 = time.time()
+    exprs = workloads.uneven_dags(T, n, seed)
+    eset = bm.ExpressionSet(T, exprs)
+    total_bytes = eset.bytes
+    log("c5: %d expressions, %.2f GB, generated and uploaded in %.1f s" % (len(exprs), total_bytes / 1e9,
+                                                                          time.time() - t0))
+    res = {"expressions": len(exprs), "bytes": total_bytes}
+    cfgs = cm.configs()
+    variants = sorted({(c["vector_width"], c["items_per_thread"]) for c in cfgs})
+    t0 = time.time()
+    with concurrent.futures.ThreadPoolExecutor(len(variants)) as ex:
+        futs = {vu: ex.submit(bm.compile_many, T, exprs, [{"vector_width": vu[0], "items_per_thread": vu[1],
+                                                           "block_size": 256}] * len(exprs)) for vu in variants}
+        fused = {vu: f.result() for vu, f in futs.items()}
+    res["compile_s_all_variants"] = time.time() - t0
+    log("c5: compiled %d variants x %d expressions in %.1f s" % (len(variants), len(exprs), res["compile_s_all_variants"]))
+    model_path = os.path.join(ROOT, "compute.scala_amd", "compute", "data", "costmodel.npz")
+    default = {"vector_width": 4, "items_per_thread": 1, "block_size": 256}
+    if sweep:
+        t0 = time.time()
+        times = cm.measure(eset, fused, cfgs, progress=lambda i: log("c5: autotuned %d/%d expressions" % (i, len(exprs)))
+                           if i % 64 == 0 else None)
+        res["timings"] = len(times)
+        log("c5: %d autotuner timings in %.1f s" % (len(times), time.time() - t0))
+        ntrain = (3 * len(exprs)) // 4
+        X, y = [], []
+        for (i, ci), us in times.items():
+            if i < ntrain:
+                X.append(cm.features(exprs[i], cfgs[ci]))
+                y.append(us)
+        model = cm.CostModel(T, len(X[0]))
+        t0 = time.time()
+        hist = model.fit(np.stack(X), np.array(y), epochs=150, batch=512)
+        res["train_s"] = time.time() - t0
+        res["train_loss_first_last"] = [hist[0], hist[-1]]
+        best_t = model_t = default_t = 0.0
+        di = cfgs.index(default)
+        for i in range(ntrain, len(exprs)):
+            ts = [times[(i, ci)] for ci in range(len(cfgs))]
+            choice = model.choose(exprs[i], cfgs)
+            best_t += min(ts)
+            model_t += ts[cfgs.index(choice)]
+            default_t += ts[di]
+        res["held_out"] = len(exprs) - ntrain
+        res["regret_vs_exhaustive"] = model_t / best_t - 1.0
+        res["gain_over_default"] = default_t / model_t
+        res["held_out_us"] = {"exhaustive": best_t, "model": model_t, "default": default_t}
+        log("c5: cost model regret %.3f, gain over default %.3f" % (res["regret_vs_exhaustive"],
+                                                                   res["gain_over_default"]))
+        os.makedirs(os.path.dirname(model_path), exist_ok=True)
+        np.savez(model_path, params=model.net.get_params(), mean=model.mean, std=model.std,
+                 ymean=np.array([model.ymean]), widths=np.array(model.net.widths))
+        if out_dir:
+            with open(os.path.join(out_dir, "c5_timings.json"), "w") as f:
+                json.dump({"configs": cfgs, "times": [[i, ci, us] for (i, ci), us in sorted(times.items())]}, f)
+        model.release()
+    # configuration per expression: cost model if trained parameters exist, else default
+    chosen = [default] * len(exprs)
+    if os.path.exists(model_path):
+        d = np.load(model_path)
+        model = cm.CostModel(T, int(d["widths"][0]), hidden=int(d["widths"][1]), layers=len(d["widths"]) - 2)
+        model.net.set_params(d["params"])
+        model.mean, model.std, model.ymean = d["mean"], d["std"], float(d["ymean"][0])
+        b256 = [c for c in cfgs if c["block_size"] == 256]
+        chosen = [model.choose(e, b256) for e in exprs]
+        model.release()
+        res["configs_from"] = "cost model"
+    else:
+        res["configs_from"] = "default"
+
+    def graph_each(cf):
+        return bm.Graph(T, lambda: [eset.launch(i, fused[(c["vector_width"], c["items_per_thread"])][i],
+                                                c["block_size"]) for i, c in enumerate(cf)])
+
+    g_def = graph_each([default] * len(exprs))
+    us_def = g_def.time_us(5)
+    g_def.release()
+    g_ch = graph_each(chosen)
+    us_each = g_ch.time_us(5)
+    g_ch.release()
+    eset.clear_outputs()
+    sel = [fused[(c["vector_width"], c["items_per_thread"])][i] for i, c in enumerate(chosen)]
+    for i, f in enumerate(sel):
+        eset.launch(i, f, 256)
+    each_out = eset.outputs()
+    eset.clear_outputs()
+    t0 = time.time()
+    b = bm.Batch(eset, sel, block=256)
+    res["batch_compile_s"] = time.time() - t0
+    gb = bm.Graph(T, b.launch)
+    us_batch = gb.time_us(5)
+    batch_out = eset.outputs()
+    res["batch_bit_identical"] = all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
+                                     for x, y in zip(each_out, batch_out))
+    gb.release()
+    res["batch_info"] = b.info()
+    b.release()
+    res["per_expression_default_us"] = us_def
+    res["per_expression_us"] = us_each
+    res["batched_us"] = us_batch
+    res["per_expression_gbs"] = total_bytes / (us_each * 1e-6) / 1e9
+    res["batched_gbs"] = total_bytes / (us_batch * 1e-6) / 1e9
+    res["batched_over_per_expression"] = us_each / us_batch
+    log("c5: per-expression %.1f us (default configs %.1f us), batched %.1f us -> %.2fx, %.0f GB/s, identical=%s" %
+        (us_each, us_def, us_batch, res["batched_over_per_expression"], res["batched_gbs"], res["batch_bit_identical"]))
+    for fs in fused.values():
+        for f in fs:
+            capi.lib.cs_fused_release(f)
+    eset.release()
+    return res
+
+
 def cpu_baseline_c2(seconds=10.0):
     """The reference's kernel for the headline expression (refcl-host, precise mode) on all
     host cores, timed on whole launches until about `seconds` have passed."""
@@ -277,7 +398,9 @@ def main():
     ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="c2,c1,c3,c4")
+    ap.add_argument("--only", default="c2,c1,c3,c4,c5")
+    ap.add_argument("--c5-sweep", action="store_true", help="run the exhaustive autotuner and train the cost model")
+    ap.add_argument("--out-dir", default=None)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every workload here runs on a single MI355X, "
@@ -302,6 +425,8 @@ def main():
                                                      ("bf16x3", capi.GEMM_BF16X3), ("fp8", capi.GEMM_FP8)])
         g = also["c4_gemm_4096"]
         g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
+    if "c5" in only:
+        also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_c2()

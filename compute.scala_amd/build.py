@@ -17,7 +17,7 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libcsmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip", "mathcheck.hip"]
+SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip", "mathcheck.hip", "mlp.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

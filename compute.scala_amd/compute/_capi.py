@@ -44,6 +44,18 @@ class CsFusedOptions(ctypes.Structure):
     ]
 
 
+class CsFusedDesc(ctypes.Structure):
+    _fields_ = [
+        ("nodes", ctypes.POINTER(CsNode)),
+        ("nnodes", ctypes.c_int),
+        ("outputs", ctypes.POINTER(ctypes.c_int32)),
+        ("noutputs", ctypes.c_int),
+        ("out_rank", ctypes.c_int),
+        ("out_shape", ctypes.POINTER(ctypes.c_int32)),
+        ("options", CsFusedOptions),
+    ]
+
+
 # cs_op
 OP_LITERAL, OP_PARAMETER, OP_TRANSFORM, OP_FILL, OP_EXTRACT = 1, 2, 3, 4, 5
 OP_NEG, OP_ABS, OP_SQRT, OP_TANH, OP_EXP, OP_LOG = 10, 11, 12, 13, 14, 15
@@ -187,10 +199,19 @@ _PROTOS = {
     "cs_fused_lookup": (I, [P, ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I,
                             ctypes.POINTER(I32), ctypes.POINTER(CsFusedOptions), ctypes.POINTER(I)]),
     "cs_fused_cache_clear": (I, [P]),
+    "cs_fused_compile_many": (I, [P, ctypes.POINTER(CsFusedDesc), I, PP]),
+    "cs_fused_launch_block": (I, [P, P, PP, I, P, I]),
     "cs_fused_generate_source": (I, [ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I,
                                      ctypes.POINTER(I32), ctypes.POINTER(CsFusedOptions), PP,
                                      ctypes.POINTER(U64)]),
     "cs_program_check": (I, [CP, CP, PP]),
+    "cs_fused_precompile": (I, [ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I, ctypes.POINTER(I32),
+                                ctypes.POINTER(CsFusedOptions)]),
+    "cs_batch_create": (I, [P, PP, I, I, PP]),
+    "cs_batch_bind": (I, [P, P, PP, I]),
+    "cs_batch_launch": (I, [P, P]),
+    "cs_batch_info": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(I), ctypes.POINTER(I)]),
+    "cs_batch_release": (I, [P]),
     "cs_reduce_sum": (I, [P, P, I64, P, P]),
     "cs_reduce_max": (I, [P, P, I64, P]),
     "cs_reduce_rows": (I, [P, P, I64, I64, P, P, P, P]),
@@ -201,6 +222,14 @@ _PROTOS = {
     "cs_fill_normal": (I, [P, P, I64, U32]),
     "cs_gemm": (I, [P, I, I, I, I, P, I, P, I, P, I]),
     "cs_math_check": (I, [P, I, I, U32, U32, P]),
+    "cs_mlp_create": (I, [P, ctypes.POINTER(I32), I, U32, PP]),
+    "cs_mlp_param_count": (I, [P, ctypes.POINTER(I64)]),
+    "cs_mlp_get_params": (I, [P, P, P, I64]),
+    "cs_mlp_set_params": (I, [P, P, P, I64]),
+    "cs_mlp_forward": (I, [P, P, P, I64, P]),
+    "cs_mlp_backward": (I, [P, P, P, P, I64, P, P]),
+    "cs_mlp_adam": (I, [P, P, P, F, I]),
+    "cs_mlp_release": (I, [P]),
 }
 
 for _name, (_res, _args) in _PROTOS.items():

@@ -1,0 +1,177 @@
+"""Learned cost model for fused-expression kernels.
+
+A small MLP (features -> 64 -> 64 -> 64 -> 1, FP32, ReLU) predicts log(microseconds) of one
+launch of a fused kernel from features of its expression DAG and its launch configuration
+(vector width V, vectors per thread U, workgroup size). It is trained on the device by the
+library's own forward, backward and Adam kernels (cs_mlp_*) on timings that the exhaustive
+autotuner collects, and it picks each expression's configuration; exhaustive search over the
+same candidates remains the fallback and the judge of the model's regret.
+"""
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi as capi
+from . import batch as batchmod
+from ._capi import check, lib
+
+VECTOR_WIDTHS = (1, 2, 4)
+ITEMS = (1, 2, 4)
+BLOCKS = (64, 128, 256, 512, 1024)
+OPS = ("+", "-", "*", "/", "exp", "tanh", "max")
+
+
+def configs():
+    return [{"vector_width": v, "items_per_thread": u, "block_size": b}
+            for v in VECTOR_WIDTHS for u in ITEMS for b in BLOCKS]
+
+
+def features(expr, cfg):
+    """Numeric description of (expression, launch configuration)."""
+    shape = expr.shape or (1,)
+    v = batchmod.effective_vector_width(expr.shape, cfg["vector_width"])
+    u, b = cfg["items_per_thread"], cfg["block_size"]
+    n = expr.out_elements
+    vectors = max(n // v, 1)
+    blocks = max(1.0, vectors / float(b * u))
+    f = [
+        math.log2(max(n, 1)),
+        float(len(shape)),
+        math.log2(max(shape[-1], 1)),
+        1.0 if shape[-1] % 4 == 0 else 0.0,
+        float(len(expr.leaves)),
+        float(sum(k == "direct" for k in expr.view_kinds)),
+        float(sum(k == "permute" for k in expr.view_kinds)),
+        float(sum(k == "broadcast" for k in expr.view_kinds)),
+        math.log2(max(expr.leaf_elements, 1)),
+        float(len(expr.ops)),
+    ]
+    f += [float(expr.ops.count(o)) for o in OPS]
+    f += [float(v), float(u), math.log2(b), math.log2(blocks), math.log2(blocks / 2048.0 + 1.0),
+          1.0 if v != cfg["vector_width"] else 0.0]
+    return np.array(f, dtype=np.float32)
+
+
+class Network(object):
+    """The device MLP (cs_mlp)."""
+
+    def __init__(self, tensors, widths, seed=1):
+        self.T = tensors
+        w = (ctypes.c_int32 * len(widths))(*widths)
+        h = ctypes.c_void_p()
+        check(lib.cs_mlp_create(tensors.device, w, len(widths), seed, ctypes.byref(h)), "cs_mlp_create")
+        self.handle = h.value
+        self.widths = list(widths)
+        c = ctypes.c_int64()
+        check(lib.cs_mlp_param_count(self.handle, ctypes.byref(c)))
+        self.count = c.value
+        self.step = 0
+        self._grads = tensors._alloc(self.count * 4)
+        self._loss = tensors._alloc(4)
+
+    def get_params(self):
+        out = np.empty(self.count, np.float32)
+        check(lib.cs_mlp_get_params(self.T.stream, self.handle, out.ctypes.data_as(ctypes.c_void_p), self.count))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        check(lib.cs_mlp_set_params(self.T.stream, self.handle, p.ctypes.data_as(ctypes.c_void_p), self.count))
+
+    def forward_device(self, x_ptr, n, y_ptr):
+        check(lib.cs_mlp_forward(self.T.stream, self.handle, x_ptr, n, y_ptr))
+
+    def gradients_device(self, x_ptr, t_ptr, n):
+        """Runs the backward pass; returns (loss, gradient buffer)."""
+        check(lib.cs_mlp_backward(self.T.stream, self.handle, x_ptr, t_ptr, n, self._loss.ptr, self._grads.ptr))
+        return self._loss, self._grads
+
+    def adam(self, lr):
+        self.step += 1
+        check(lib.cs_mlp_adam(self.T.stream, self.handle, self._grads.ptr, lr, self.step))
+
+    def release(self):
+        if self.handle:
+            lib.cs_mlp_release(self.handle)
+            self._grads.release()
+            self._loss.release()
+            self.handle = None
+
+
+class CostModel(object):
+    """Predicts log-microseconds; selects the fastest predicted configuration."""
+
+    def __init__(self, tensors, nfeatures, hidden=64, layers=3, seed=1):
+        self.T = tensors
+        self.net = Network(tensors, [nfeatures] + [hidden] * layers + [1], seed)
+        self.mean = np.zeros(nfeatures, np.float32)
+        self.std = np.ones(nfeatures, np.float32)
+        self.ymean = 0.0
+        self.history = []
+
+    def _upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        b = self.T._alloc(max(a.nbytes, 4))
+        check(lib.cs_buffer_write(self.T.stream, b.handle, 0, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1))
+        return b
+
+    def fit(self, X, us, epochs=300, batch=512, lr=2e-3, seed=0):
+        """Trains on features X[n, f] and measured microseconds; returns the loss history."""
+        X = np.asarray(X, np.float32)
+        y = np.log(np.asarray(us, np.float64)).astype(np.float32)
+        self.mean = X.mean(axis=0)
+        self.std = X.std(axis=0) + 1e-3
+        self.ymean = float(y.mean())
+        Xn = (X - self.mean) / self.std
+        yn = y - self.ymean
+        rng = np.random.default_rng(seed)
+        n = len(Xn)
+        for ep in range(epochs):
+            perm = rng.permutation(n)
+            xb = self._upload(Xn[perm])
+            yb = self._upload(yn[perm])
+            losses = []
+            for s in range(0, n - batch + 1, batch):
+                loss_buf, _ = self.net.gradients_device(xb.ptr + s * X.shape[1] * 4, yb.ptr + s * 4, batch)
+                self.net.adam(lr * (0.5 if ep > epochs * 0.7 else 1.0))
+                if s == 0:
+                    losses.append(float(loss_buf.to_numpy(1)[0]))
+            xb.release()
+            yb.release()
+            self.history.append(float(np.mean(losses)) if losses else float("nan"))
+        return self.history
+
+    def predict_log_us(self, X):
+        X = (np.asarray(X, np.float32) - self.mean) / self.std
+        xb = self._upload(X)
+        out = self.T._alloc(len(X) * 4)
+        self.net.forward_device(xb.ptr, len(X), out.ptr)
+        y = out.to_numpy(len(X)) + self.ymean
+        xb.release()
+        out.release()
+        return y
+
+    def choose(self, expr, candidates):
+        X = np.stack([features(expr, c) for c in candidates])
+        return candidates[int(np.argmin(self.predict_log_us(X)))]
+
+    def release(self):
+        self.net.release()
+
+
+def measure(eset, fused_by_variant, cfgs, reps=3, min_us=40.0, progress=None):
+    """Exhaustive autotuner sweep: microseconds of one launch of expression i with each
+    configuration, from a graph of back-to-back launches. Returns {(i, cfg_index): us}."""
+    out = {}
+    for i, e in enumerate(eset.exprs):
+        if progress:
+            progress(i)
+        for ci, cfg in enumerate(cfgs):
+            f = fused_by_variant[(cfg["vector_width"], cfg["items_per_thread"])][i]
+            est = max(2.0, e.bytes / 4e6)  # ~4 TB/s guess to size the repetition count
+            k = int(min(64, max(4, min_us / est)))
+            g = batchmod.Graph(eset.T, lambda: [eset.launch(i, f, cfg["block_size"]) for _ in range(k)])
+            out[(i, ci)] = g.time_us(reps) / k
+            g.release()
+    return out

@@ -1,0 +1,125 @@
+"""Seeded expression workloads.
+
+`uneven_dags(T, n, seed)` builds the batch of uneven fused expressions: each expression has
+3 to 12 operations drawn from {+, -, *, /, exp, tanh, max}, an output of log-uniform size in
+[1e3, 4e6] FP32 elements factored into a rank-1 to rank-4 shape, and 2 to 4 distinct leaves,
+each read directly, through a permute view or through a broadcast view (the reference's
+leading-dimension broadcast, Tensors.scala:816-855). Leaf data are U(-1, 1) from the Wang
+hash (one seed per leaf). Everything is determined by `seed`.
+"""
+import math
+
+import numpy as np
+
+
+class Expr(object):
+    """One expression of a batch: its tensor, leaf data and byte count."""
+
+    def __init__(self, index, tensor, leaves, leaf_data, ops, view_kinds):
+        self.index = index
+        self.tensor = tensor
+        self.shape = tuple(tensor.shape)
+        self.leaves = leaves            # leaf tensors in argument order
+        self.leaf_data = leaf_data      # numpy arrays, same order
+        self.ops = ops                  # op names in creation order
+        self.view_kinds = view_kinds    # "direct" / "permute" / "broadcast" per leaf
+        self.out_elements = int(np.prod(self.shape)) if self.shape else 1
+        self.leaf_elements = int(sum(d.size for d in leaf_data))
+
+    @property
+    def bytes(self):
+        """Algorithmic traffic: every distinct leaf read once, the output written once."""
+        return 4 * (self.leaf_elements + self.out_elements)
+
+
+def _factor(n, rank, rng):
+    """A shape of `rank` dimensions whose product is close to n (innermost often even)."""
+    if rank == 1:
+        return (max(1, n),)
+    w = rng.dirichlet(np.ones(rank))
+    logs = w * math.log(max(n, 1))
+    dims = [max(1, int(round(math.exp(x)))) for x in logs]
+    if rng.random() < 0.7 and dims[-1] > 1:
+        dims[-1] = max(2, dims[-1] // 2 * 2)
+    return tuple(dims)
+
+
+def _uniform(n, seed):
+    """U(-1, 1) from the Wang hash, FP64 then rounded (the synthetic-input definition)."""
+    i = np.arange(n, dtype=np.uint64).astype(np.uint32) ^ np.uint32(seed & 0xFFFFFFFF)
+    with np.errstate(over="ignore"):
+        v = (i ^ np.uint32(61)) ^ (i >> np.uint32(16))
+        v = v * np.uint32(9)
+        v = v ^ (v << np.uint32(4))
+        v = v * np.uint32(0x27D4EB2D)
+        v = v ^ (v >> np.uint32(15))
+    return (-1.0 + 2.0 * (v.astype(np.float64) * (1.0 / 4294967296.0))).astype(np.float32)
+
+
+OPS = ("+", "-", "*", "/", "exp", "tanh", "max")
+
+
+def uneven_dags(T, n=512, seed=5001, min_elements=1_000, max_elements=4_000_000):
+    rng = np.random.default_rng(seed)
+    exprs = []
+    for e in range(n):
+        size = int(round(math.exp(rng.uniform(math.log(min_elements), math.log(max_elements)))))
+        rank = int(rng.integers(1, 5))
+        shape = _factor(size, rank, rng)
+        nleaves = int(rng.integers(2, 5))
+        leaves, data, views, kinds = [], [], [], []
+        for k in range(nleaves):
+            kind = rng.choice(["direct", "permute", "broadcast"], p=[0.4, 0.3, 0.3])
+            if kind == "permute" and rank > 1:
+                perm = list(rng.permutation(rank))
+                src_shape = tuple(shape[p] for p in perm)
+                inv = list(np.argsort(perm))
+                arr = _uniform(int(np.prod(src_shape)), seed * 1000 + e * 10 + k).reshape(src_shape)
+                t = T.Tensor(arr)
+                views.append(t.permute(inv))
+            elif kind == "broadcast" and rank > 1:
+                keep = int(rng.integers(1, rank))  # leading dimensions kept, the rest appended
+                src_shape = tuple(shape[:keep])
+                if rng.random() < 0.5:  # a size-1 dimension inside the kept prefix
+                    j = int(rng.integers(0, keep))
+                    src_shape = tuple(1 if i == j else s for i, s in enumerate(src_shape))
+                arr = _uniform(int(np.prod(src_shape)), seed * 1000 + e * 10 + k).reshape(src_shape)
+                t = T.Tensor(arr)
+                views.append(t.broadcast(shape))
+            else:
+                kind = "direct"
+                arr = _uniform(int(np.prod(shape)), seed * 1000 + e * 10 + k).reshape(shape)
+                t = T.Tensor(arr)
+                views.append(t)
+            leaves.append(t)
+            data.append(arr)
+            kinds.append(str(kind))
+        nops = int(rng.integers(3, 13))
+        ops = []
+        expr = views[0]
+        pool = list(views)
+        for _ in range(nops):
+            op = OPS[int(rng.integers(0, len(OPS)))]
+            ops.append(op)
+            if op == "exp":
+                expr = T.Tensor.exp(expr)
+            elif op == "tanh":
+                expr = T.Tensor.tanh(expr)
+            else:
+                other = pool[int(rng.integers(0, len(pool)))]
+                if rng.random() < 0.5:
+                    a, b = expr, other
+                else:
+                    a, b = other, expr
+                if op == "max":
+                    expr = T.Tensor.max(a, b)
+                else:
+                    expr = {"+": a.__add__, "-": a.__sub__, "*": a.__mul__, "/": a.__truediv__}[op](b)
+            pool.append(expr)
+        # arguments in first-visit order of the closure (the kernel's leaf order)
+        from . import trees
+        order = [p.id for p in trees.parameters(expr.closure)]
+        used = [leaves.index(t) for t in order]
+        exprs.append(Expr(e, expr, [leaves[i] for i in used], [data[i] for i in used], ops,
+                          [kinds[i] for i in used]))
+    return exprs

@@ -17,20 +17,6 @@
 #include "internal.h"
 #include "precise_math.inc"  // kPreciseMathSource, generated from precise_math.h
 
-struct cs_fused {
-  std::atomic<int> refs{1};
-  cs_device *device = nullptr;
-  std::string key;
-  uint64_t hash = 0;
-  std::string source;
-  hipModule_t module = nullptr;
-  hipFunction_t function = nullptr;
-  int nleaves = 0;
-  int64_t out_elements = 0;  // coordinates x tuple width
-  int64_t vectors = 0;       // number of V-wide output vectors
-  uint32_t block = 256;
-  uint32_t grid = 0;
-};
 
 namespace csmi {
 namespace {
@@ -336,33 +322,73 @@ class Emitter {
  public:
   Emitter(const Dag &dag, const Options &o) : dag_(dag), o_(o) {}
 
-  std::string generate(int64_t *vectors_out) {
-    const int r = static_cast<int>(dag_.out_shape.size());
-    const int V = o_.V, U = o_.U;
-    int64_t total = 1;
-    for (auto s : dag_.out_shape) total *= s;
-    const int64_t vectors = total / V;
-    *vectors_out = vectors;
-    const int nout = static_cast<int>(dag_.outputs.size());
-
+  // The whole kernel source: a stand-alone kernel over blockIdx.x.
+  std::string generate(int64_t *vectors_out, const std::string &name = "jit_kernel", bool with_prelude = true) {
     std::ostringstream s;
+    header(s, with_prelude);
+    s << "extern \"C\" __global__ __launch_bounds__(1024) void " << name << "(";
+    params(s);
+    s << ") {\n";
+    body(s, "blockIdx.x", vectors_out);
+    s << "}\n";
+    return s.str();
+  }
+
+  // The same body as a device function over an explicit tile index, for batches.
+  std::string device_function(const std::string &name, int64_t *vectors_out) {
+    std::ostringstream s;
+    s << "__device__ __forceinline__ void " << name << "(const unsigned tile, ";
+    params(s);
+    s << ") {\n";
+    body(s, "tile", vectors_out);
+    s << "}\n";
+    return s.str();
+  }
+
+  // Typedefs (and the accurate math functions) the bodies need, once per module.
+  static std::string prelude(int math_mode) {
+    std::string p;
+    if (math_mode == CS_MATH_ACCURATE) p += std::string(kPreciseMathSource) + "\n";
+    p += "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
+    p += "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
+    p += "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
+    return p;
+  }
+
+  int nleaves() const { return dag_.nleaves; }
+
+ private:
+  void header(std::ostringstream &s, bool with_prelude) {
+    const int r = static_cast<int>(dag_.out_shape.size());
     s << "// Generated by csmi for gfx950: " << dag_.nodes.size() << " nodes, " << dag_.nleaves
       << " leaves, output [";
     for (int i = 0; i < r; ++i) s << (i ? "," : "") << dag_.out_shape[i];
-    s << "] x " << nout << ", V=" << V << " U=" << U << " block=" << o_.block << "\n";
-    if (o_.math_mode == CS_MATH_ACCURATE) s << kPreciseMathSource << "\n";
-    s << "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
-    s << "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
-    s << "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
-    s << "extern \"C\" __global__ __launch_bounds__(" << o_.block << ") void jit_kernel(";
+    s << "] x " << dag_.outputs.size() << ", V=" << o_.V << " U=" << o_.U << "\n";
+    if (with_prelude) s << prelude(o_.math_mode);
+  }
+
+  void params(std::ostringstream &s) {
     for (int i = 0; i < dag_.nleaves; ++i) s << "const float* __restrict__ p" << i << ", ";
-    s << "float* __restrict__ out) {\n";
-    s << "  const unsigned vbase = blockIdx.x * " << (int64_t(o_.block) * U) << "u + threadIdx.x;\n";
+    s << "float* __restrict__ out";
+  }
+
+  // tile: the index of the block's slice of blockDim.x * U vectors.
+  void body(std::ostringstream &s, const std::string &tile, int64_t *vectors_out) {
+    const int r = static_cast<int>(dag_.out_shape.size());
+    const int V = o_.V, U = o_.U;
+    int64_t total = 1;
+    for (auto d : dag_.out_shape) total *= d;
+    const int64_t vectors = total / V;
+    *vectors_out = vectors;
+    const int nout = static_cast<int>(dag_.outputs.size());
+    uniform_.clear();
+    fill_of_.clear();
+    s << "  const unsigned vbase = (" << tile << ") * (blockDim.x * " << U << "u) + threadIdx.x;\n";
     // phase 0: vector index, validity and coordinates of element 0 of every vector
     s << "  unsigned q[" << U << "]; bool live[" << U << "];\n";
     for (int d = 0; d < r; ++d) s << "  int c" << d << "[" << U << "];\n";
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
-    s << "    q[u] = vbase + u * " << o_.block << "u;\n";
+    s << "    q[u] = vbase + u * blockDim.x;\n";
     s << "    live[u] = q[u] < " << vectors << "u;\n";
     s << "    unsigned rest = q[u] * " << V << "u;\n";
     for (int d = r - 1; d >= 0; --d) {
@@ -402,11 +428,9 @@ class Emitter {
         i += 1;
       }
     }
-    s << "  }\n}\n";
-    return s.str();
+    s << "  }\n";
   }
 
- private:
   std::string store(const std::string &addr, const std::string &val, const std::string &type) {
     if (o_.nt) return "__builtin_nontemporal_store(" + val + ", (" + type + "*)(" + addr + "));";
     return "*(" + type + "*)(" + addr + ") = " + val + ";";
@@ -688,6 +712,63 @@ std::vector<std::string> math_options(int mode) {
 
 using namespace csmi;
 
+// A loaded module that several compiled expressions may share (cs_fused_compile_many).
+struct ModuleRef {
+  hipModule_t module = nullptr;
+  int ordinal = 0;
+  ~ModuleRef() {
+    if (module) {
+      (void)hipSetDevice(ordinal);
+      (void)hipModuleUnload(module);
+    }
+  }
+};
+
+struct cs_fused {
+  std::atomic<int> refs{1};
+  cs_device *device = nullptr;
+  std::string key;
+  uint64_t hash = 0;
+  std::string source;
+  std::shared_ptr<ModuleRef> module;
+  hipFunction_t function = nullptr;
+  int nleaves = 0;
+  int64_t out_elements = 0;  // coordinates x tuple width
+  int64_t vectors = 0;       // number of V-wide output vectors
+  uint32_t block = 256;
+  uint32_t grid = 0;
+  Dag dag;                   // kept for batches, which re-emit the body
+  Options opts;
+};
+
+struct cs_batch {
+  cs_device *device = nullptr;
+  std::vector<cs_fused *> exprs;
+  std::string source;
+  hipModule_t module = nullptr;
+  hipFunction_t kernel = nullptr, reset = nullptr;
+  uint32_t block = 256;
+  uint32_t grid = 0;
+  int64_t items = 0;
+  int max_args = 0;
+  int total_args = 0;
+  void *items_dev = nullptr;  // uint2 {expr, tile} per work item
+  void *args_dev = nullptr;   // exprs x max_args device pointers
+  unsigned *counter = nullptr;
+};
+
+namespace {
+
+Dag prepare(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutputs, int out_rank,
+            const int32_t *out_shape, const cs_fused_options *options, Options *o, std::string *key) {
+  Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
+  *o = resolve_options(options, dag);
+  *key = canonical_key(dag, *o);
+  return dag;
+}
+
+}  // namespace
+
 extern "C" {
 
 cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32_t *outputs,
@@ -695,14 +776,29 @@ cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32
                                    const cs_fused_options *options, char **source, uint64_t *key) {
   return guard([&] {
     if (!source) fail(CS_INVALID_VALUE, "source is null");
-    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
-    Options o = resolve_options(options, dag);
+    Options o;
+    std::string k;
+    Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &k);
     int64_t vectors = 0;
     std::string src = Emitter(dag, o).generate(&vectors);
     *source = static_cast<char *>(std::malloc(src.size() + 1));
     if (!*source) fail(CS_OUT_OF_HOST_MEMORY, "malloc");
     std::memcpy(*source, src.c_str(), src.size() + 1);
-    if (key) *key = fnv1a(canonical_key(dag, o));
+    if (key) *key = fnv1a(k);
+  });
+}
+
+cs_status cs_fused_precompile(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutputs,
+                              int out_rank, const int32_t *out_shape, const cs_fused_options *options) {
+  return guard([&] {
+    Options o;
+    std::string k;
+    Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &k);
+    int64_t vectors = 0;
+    std::string src = Emitter(dag, o).generate(&vectors);
+    std::string log;
+    if (!precompile(src, math_options(o.math_mode), "gfx950", &log))
+      fail(CS_BUILD_PROGRAM_FAILURE, "hiprtc build failed:\n" + log);
   });
 }
 
@@ -735,9 +831,9 @@ cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
   return guard([&] {
     if (!device) fail(CS_INVALID_DEVICE, "device is null");
     if (!fused) fail(CS_INVALID_VALUE, "fused is null");
-    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
-    Options o = resolve_options(options, dag);
-    std::string key = canonical_key(dag, o);
+    Options o;
+    std::string key;
+    Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &key);
     {
       std::lock_guard<std::recursive_mutex> l(device->mu);
       auto it = device->fused_cache.find(key);
@@ -750,26 +846,29 @@ cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
     int64_t vectors = 0;
     std::string src = Emitter(dag, o).generate(&vectors);
     std::string log;
-    hipModule_t m = build_module(device, src, math_options(o.math_mode), &log);
+    auto mod = std::make_shared<ModuleRef>();
+    mod->ordinal = device->ordinal;
+    mod->module = build_module(device, src, math_options(o.math_mode), &log);
     auto *f = new cs_fused();
     f->device = device;
     f->key = key;
     f->hash = fnv1a(key);
     f->source = src;
-    f->module = m;
-    CS_HIP(hipModuleGetFunction(&f->function, m, "jit_kernel"));
+    f->module = mod;
+    CS_HIP(hipModuleGetFunction(&f->function, mod->module, "jit_kernel"));
     f->nleaves = dag.nleaves;
     int64_t total = 1;
-    for (auto s : dag.out_shape) total *= s;
+    for (auto d : dag.out_shape) total *= d;
     f->out_elements = total * static_cast<int64_t>(dag.outputs.size());
     f->vectors = vectors;
     f->block = static_cast<uint32_t>(o.block);
     int64_t per_block = int64_t(o.block) * o.U;
     f->grid = static_cast<uint32_t>((vectors + per_block - 1) / per_block);
+    f->dag = std::move(dag);
+    f->opts = o;
     std::lock_guard<std::recursive_mutex> l(device->mu);
     auto it = device->fused_cache.find(key);
     if (it != device->fused_cache.end()) {  // another thread won the race
-      (void)hipModuleUnload(m);
       delete f;
       it->second->refs++;
       *fused = it->second;
@@ -778,6 +877,122 @@ cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
     f->refs = 2;  // one for the cache, one for the caller
     device->fused_cache.emplace(key, f);
     *fused = f;
+  });
+}
+
+cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out) {
+  return guard([&] {
+    if (!device) fail(CS_INVALID_DEVICE, "device is null");
+    if (n < 0 || (n > 0 && (!descs || !out))) fail(CS_INVALID_VALUE, "null argument");
+    std::vector<Dag> dags(static_cast<size_t>(n));
+    std::vector<Options> opts(static_cast<size_t>(n));
+    std::vector<std::string> keys(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) {
+      const cs_fused_desc &d = descs[i];
+      dags[i] = prepare(d.nodes, d.nnodes, d.outputs, d.noutputs, d.out_rank, d.out_shape, &d.options, &opts[i],
+                        &keys[i]);
+      out[i] = nullptr;
+    }
+    // Cached expressions come from the cache; the rest compile together, one module per
+    // math mode (the modes differ in compiler options).
+    std::map<int, std::vector<int>> todo;
+    {
+      std::lock_guard<std::recursive_mutex> l(device->mu);
+      for (int i = 0; i < n; ++i) {
+        auto it = device->fused_cache.find(keys[i]);
+        if (it != device->fused_cache.end()) {
+          it->second->refs++;
+          out[i] = it->second;
+        } else {
+          bool dup = false;
+          for (int j : todo[opts[i].math_mode]) dup = dup || keys[j] == keys[i];
+          if (!dup) todo[opts[i].math_mode].push_back(i);
+        }
+      }
+    }
+    for (auto &group : todo) {
+      std::ostringstream src;
+      src << Emitter::prelude(group.first);
+      std::vector<int64_t> vectors(group.second.size());
+      for (size_t k = 0; k < group.second.size(); ++k) {
+        int i = group.second[k];
+        src << Emitter(dags[i], opts[i]).generate(&vectors[k], "jit_kernel_" + std::to_string(k), false);
+      }
+      std::string log;
+      auto mod = std::make_shared<ModuleRef>();
+      mod->ordinal = device->ordinal;
+      mod->module = build_module(device, src.str(), math_options(group.first), &log);
+      for (size_t k = 0; k < group.second.size(); ++k) {
+        int i = group.second[k];
+        auto *f = new cs_fused();
+        f->device = device;
+        f->key = keys[i];
+        f->hash = fnv1a(keys[i]);
+        int64_t v = 0;
+        f->source = Emitter(dags[i], opts[i]).generate(&v);
+        f->module = mod;
+        CS_HIP(hipModuleGetFunction(&f->function, mod->module, ("jit_kernel_" + std::to_string(k)).c_str()));
+        f->nleaves = dags[i].nleaves;
+        int64_t total = 1;
+        for (auto d : dags[i].out_shape) total *= d;
+        f->out_elements = total * static_cast<int64_t>(dags[i].outputs.size());
+        f->vectors = vectors[k];
+        f->block = static_cast<uint32_t>(opts[i].block);
+        int64_t per_block = int64_t(opts[i].block) * opts[i].U;
+        f->grid = static_cast<uint32_t>((vectors[k] + per_block - 1) / per_block);
+        f->dag = dags[i];
+        f->opts = opts[i];
+        std::lock_guard<std::recursive_mutex> l(device->mu);
+        auto it = device->fused_cache.find(keys[i]);
+        if (it != device->fused_cache.end()) {
+          delete f;
+          it->second->refs++;
+          out[i] = it->second;
+        } else {
+          f->refs = 2;
+          device->fused_cache.emplace(keys[i], f);
+          out[i] = f;
+        }
+      }
+    }
+    // duplicates within the request resolve to the now-cached kernel
+    std::lock_guard<std::recursive_mutex> l(device->mu);
+    for (int i = 0; i < n; ++i)
+      if (!out[i]) {
+        auto it = device->fused_cache.find(keys[i]);
+        if (it == device->fused_cache.end()) fail(CS_INVALID_OPERATION, "expression missing after compile");
+        it->second->refs++;
+        out[i] = it->second;
+      }
+  });
+}
+
+cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
+                                void *out, int block) {
+  return guard([&] {
+    if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (block == 0) block = static_cast<int>(fused->block);
+    if (block < 64 || block > 1024 || block % 64)
+      fail(CS_INVALID_WORK_GROUP_SIZE, "block must be a multiple of 64 in 64..1024");
+    if (nleaves != fused->nleaves)
+      fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
+                                       std::to_string(nleaves));
+    int64_t per_block = int64_t(block) * fused->opts.U;
+    uint32_t grid = static_cast<uint32_t>((fused->vectors + per_block - 1) / per_block);
+    if (grid == 0) return;
+    if (!out) fail(CS_INVALID_KERNEL_ARGS, "output pointer is null");
+    std::vector<const void *> vals(static_cast<size_t>(nleaves) + 1);
+    std::vector<void *> args(static_cast<size_t>(nleaves) + 1);
+    for (int i = 0; i < nleaves; ++i) {
+      if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
+      vals[i] = leaves[i];
+      args[i] = &vals[i];
+    }
+    vals[nleaves] = out;
+    args[nleaves] = &vals[nleaves];
+    set_device(stream->device);
+    CS_HIP(hipModuleLaunchKernel(fused->function, grid, 1, 1, static_cast<uint32_t>(block), 1, 1, 0,
+                                 stream->stream, args.data(), nullptr));
   });
 }
 
@@ -791,11 +1006,7 @@ cs_status cs_fused_retain(cs_fused *fused) {
 cs_status cs_fused_release(cs_fused *fused) {
   return guard([&] {
     if (!fused) fail(CS_INVALID_VALUE, "fused is null");
-    if (--fused->refs == 0) {
-      set_device(fused->device);
-      (void)hipModuleUnload(fused->module);
-      delete fused;
-    }
+    if (--fused->refs == 0) delete fused;  // the module goes with its last expression
   });
 }
 
@@ -850,9 +1061,9 @@ cs_status cs_fused_lookup(cs_device *device, const cs_node *nodes, int nnodes, c
                           const cs_fused_options *options, int *hit) {
   return guard([&] {
     if (!device || !hit) fail(CS_INVALID_VALUE, "null argument");
-    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
-    Options o = resolve_options(options, dag);
-    std::string key = canonical_key(dag, o);
+    Options o;
+    std::string key;
+    prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &key);
     std::lock_guard<std::recursive_mutex> l(device->mu);
     *hit = device->fused_cache.count(key) ? 1 : 0;
   });
@@ -868,6 +1079,166 @@ cs_status cs_fused_cache_clear(cs_device *device) {
       device->fused_cache.clear();
     }
     for (cs_fused *f : drop) cs_fused_release(f);
+  });
+}
+
+// ------------------------------------------------------------------------ batches
+// One persistent kernel for many fused expressions: every expression's body becomes a
+// device function over a tile index (its own vector width and vectors per thread), and a
+// queue of (expression, tile) work items, ordered by decreasing expression cost, is drained
+// by a fixed grid of workgroups through an atomic counter. A reset kernel zeroes the counter
+// before each run, so a captured graph replays correctly.
+cs_status cs_batch_create(cs_device *device, cs_fused *const *exprs, int n, int block, cs_batch **batch) {
+  return guard([&] {
+    if (!device || !batch) fail(CS_INVALID_VALUE, "null argument");
+    if (n <= 0 || !exprs) fail(CS_INVALID_VALUE, "empty batch");
+    if (block < 64 || block > 1024 || block % 64) fail(CS_INVALID_WORK_GROUP_SIZE, "block must be 64..1024");
+    int math_mode = exprs[0]->opts.math_mode;
+    for (int i = 0; i < n; ++i) {
+      if (!exprs[i]) fail(CS_INVALID_VALUE, "null expression");
+      if (exprs[i]->device != device) fail(CS_INVALID_DEVICE, "expression compiled on another device");
+      if (exprs[i]->opts.math_mode != math_mode) fail(CS_INVALID_VALUE, "mixed math modes in one batch");
+    }
+    auto *b = new cs_batch();
+    b->device = device;
+    b->block = static_cast<uint32_t>(block);
+    std::ostringstream src;
+    src << "// csmi batch of " << n << " expressions\n" << Emitter::prelude(math_mode);
+    std::vector<int64_t> tiles(static_cast<size_t>(n)), tile_bytes(static_cast<size_t>(n));
+    std::vector<double> cost(static_cast<size_t>(n));
+    int max_args = 1;
+    for (int i = 0; i < n; ++i) {
+      cs_fused *f = exprs[i];
+      int64_t vectors = 0;
+      src << Emitter(f->dag, f->opts).device_function("expr_" + std::to_string(i), &vectors);
+      int64_t per = int64_t(block) * f->opts.U;
+      tiles[i] = (vectors + per - 1) / per;
+      tile_bytes[i] = per * f->opts.V * int64_t(f->dag.outputs.size() + f->nleaves) * 4;
+      cost[i] = double(vectors) * f->opts.V * (f->nleaves + 1 + double(f->dag.nodes.size()) / 4.0);
+      max_args = std::max(max_args, f->nleaves + 1);
+    }
+    b->max_args = max_args;
+    src << "extern \"C\" __global__ void batch_reset(unsigned* counter) { *counter = 0u; }\n";
+    // Work item = (expression, first tile, tile count). Thread 0 takes the next item from
+    // the counter while the block works on the current one, hiding the atomic's latency.
+    src << "extern \"C\" __global__ __launch_bounds__(1024) void batch_kernel(const uint4* __restrict__ items, "
+           "const unsigned nitems, const float* const* __restrict__ args, unsigned* counter) {\n";
+    src << "  __shared__ unsigned next;\n";
+    src << "  if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n  __syncthreads();\n";
+    src << "  unsigned w = next;\n  __syncthreads();\n";
+    src << "  while (w < nitems) {\n";
+    src << "    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n";
+    src << "    const uint4 it = items[w];\n    const float* const* a = args + it.x * " << max_args << "u;\n";
+    src << "    for (unsigned t = it.y; t < it.y + it.z; ++t) {\n";
+    src << "      switch (it.x) {\n";
+    for (int i = 0; i < n; ++i) {
+      src << "        case " << i << ": expr_" << i << "(t";
+      for (int k = 0; k < exprs[i]->nleaves; ++k) src << ", a[" << k << "]";
+      src << ", (float*)a[" << exprs[i]->nleaves << "]); break;\n";
+    }
+    src << "        default: break;\n      }\n    }\n";
+    src << "    __syncthreads();\n    w = next;\n    __syncthreads();\n  }\n}\n";
+    b->source = src.str();
+    std::string log;
+    b->module = build_module(device, b->source, math_options(math_mode), &log);
+    CS_HIP(hipModuleGetFunction(&b->kernel, b->module, "batch_kernel"));
+    CS_HIP(hipModuleGetFunction(&b->reset, b->module, "batch_reset"));
+    // Work items, largest expressions first.
+    std::vector<int> order(static_cast<size_t>(n));
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+      return cost[x] / std::max<int64_t>(tiles[x], 1) > cost[y] / std::max<int64_t>(tiles[y], 1);
+    });
+    // Runs of consecutive tiles of about kItemBytes of traffic: few enough items that the
+    // single counter is not a bottleneck, many enough to balance the grid.
+    constexpr int64_t kItemBytes = 256 << 10;
+    std::vector<uint32_t> items;
+    for (int i : order) {
+      int64_t run = std::max<int64_t>(1, kItemBytes / std::max<int64_t>(tile_bytes[i], 1));
+      for (int64_t t = 0; t < tiles[i]; t += run) {
+        items.push_back(static_cast<uint32_t>(i));
+        items.push_back(static_cast<uint32_t>(t));
+        items.push_back(static_cast<uint32_t>(std::min(run, tiles[i] - t)));
+        items.push_back(0u);
+      }
+    }
+    b->items = static_cast<int64_t>(items.size() / 4);
+    set_device(device);
+    CS_HIP(hipMalloc(&b->items_dev, std::max<size_t>(items.size() * 4, 8)));
+    if (!items.empty()) CS_HIP(hipMemcpy(b->items_dev, items.data(), items.size() * 4, hipMemcpyHostToDevice));
+    CS_HIP(hipMalloc(&b->args_dev, size_t(n) * max_args * sizeof(void *)));
+    CS_HIP(hipMemset(b->args_dev, 0, size_t(n) * max_args * sizeof(void *)));
+    CS_HIP(hipMalloc(reinterpret_cast<void **>(&b->counter), 256));
+    CS_HIP(hipMemset(b->counter, 0, 256));
+    int occupancy = 0;
+    CS_HIP(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&occupancy, b->kernel, block, 0));
+    if (occupancy < 1) occupancy = 1;
+    int64_t want = int64_t(device->props.multiProcessorCount) * occupancy;
+    b->grid = static_cast<uint32_t>(std::max<int64_t>(1, std::min<int64_t>(want, b->items)));
+    for (int i = 0; i < n; ++i) {
+      exprs[i]->refs++;
+      b->exprs.push_back(exprs[i]);
+    }
+    for (int i = 0; i < n; ++i) b->total_args += exprs[i]->nleaves + 1;
+    *batch = b;
+  });
+}
+
+cs_status cs_batch_bind(cs_stream *stream, cs_batch *batch, const void *const *args, int nargs) {
+  return guard([&] {
+    if (!stream || !batch) fail(CS_INVALID_VALUE, "null argument");
+    if (nargs != batch->total_args)
+      fail(CS_INVALID_KERNEL_ARGS, "batch takes " + std::to_string(batch->total_args) + " pointers");
+    std::vector<const void *> table(batch->exprs.size() * size_t(batch->max_args), nullptr);
+    int k = 0;
+    for (size_t i = 0; i < batch->exprs.size(); ++i)
+      for (int j = 0; j <= batch->exprs[i]->nleaves; ++j) {
+        if (!args || !args[k]) fail(CS_INVALID_KERNEL_ARGS, "null pointer in batch arguments");
+        table[i * batch->max_args + j] = args[k++];
+      }
+    set_device(stream->device);
+    CS_HIP(hipMemcpyAsync(batch->args_dev, table.data(), table.size() * sizeof(void *), hipMemcpyHostToDevice,
+                          stream->stream));
+    CS_HIP(hipStreamSynchronize(stream->stream));
+  });
+}
+
+cs_status cs_batch_launch(cs_stream *stream, cs_batch *batch) {
+  return guard([&] {
+    if (!stream || !batch) fail(CS_INVALID_VALUE, "null argument");
+    set_device(stream->device);
+    void *counter = batch->counter;
+    void *rargs[] = {&counter};
+    CS_HIP(hipModuleLaunchKernel(batch->reset, 1, 1, 1, 1, 1, 1, 0, stream->stream, rargs, nullptr));
+    if (batch->items == 0) return;
+    const void *items = batch->items_dev;
+    unsigned nitems = static_cast<unsigned>(batch->items);
+    const void *argtab = batch->args_dev;
+    void *kargs[] = {&items, &nitems, &argtab, &counter};
+    CS_HIP(hipModuleLaunchKernel(batch->kernel, batch->grid, 1, 1, batch->block, 1, 1, 0, stream->stream, kargs,
+                                 nullptr));
+  });
+}
+
+cs_status cs_batch_info(cs_batch *batch, int64_t *items, int *grid, int *block) {
+  return guard([&] {
+    if (!batch) fail(CS_INVALID_VALUE, "batch is null");
+    if (items) *items = batch->items;
+    if (grid) *grid = static_cast<int>(batch->grid);
+    if (block) *block = static_cast<int>(batch->block);
+  });
+}
+
+cs_status cs_batch_release(cs_batch *batch) {
+  return guard([&] {
+    if (!batch) fail(CS_INVALID_VALUE, "batch is null");
+    set_device(batch->device);
+    (void)hipModuleUnload(batch->module);
+    (void)hipFree(batch->items_dev);
+    (void)hipFree(batch->args_dev);
+    (void)hipFree(batch->counter);
+    for (cs_fused *f : batch->exprs) cs_fused_release(f);
+    delete batch;
   });
 }
 

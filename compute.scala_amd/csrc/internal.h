@@ -160,6 +160,7 @@ struct cs_kernel {
   std::string name;
 };
 
+struct cs_batch;
 struct cs_graph {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -179,6 +180,11 @@ hipModule_t build_module(cs_device *device, const std::string &source,
 // Compile-only check (no device needed); returns true on success and fills *log.
 bool compile_check(const std::string &source, const std::vector<std::string> &options,
                    const std::string &arch, std::string *log, size_t *code_size);
+// Compiles into the on-disk code-object cache without a device; true if cached or built.
+bool precompile(const std::string &source, const std::vector<std::string> &options, const std::string &arch,
+                std::string *log);
+// Directory of the on-disk code-object cache ("" when disabled).
+std::string kernel_cache_dir();
 // Options every HIP build in this library uses.
 std::vector<std::string> base_compile_options(const std::string &arch);
 }  // namespace csmi

@@ -1,9 +1,13 @@
 // Device runtime: devices, streams, events, the pooled arena, pinned staging, hiprtc
 // programs and graphs. Replaces the OpenCL module (OpenCL.scala:1139-1433) and the
 // Memory module (Memory.scala:13-335) for the Tensor API.
+#include <dlfcn.h>
 #include <hip/hiprtc.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <fstream>
 #include <condition_variable>
 #include <cstdio>
 #include <functional>
@@ -221,6 +225,76 @@ static bool rtc_compile(const std::string &source, const std::vector<std::string
   return ok;
 }
 
+// ------------------------------------------------------------------------ code-object cache
+// Compiled code objects are kept on disk, keyed by a hash of (arch, options, source), in
+// $CSMI_KERNEL_CACHE or, by default, the `kcache` directory next to the library's `lib`
+// directory. Generated kernels are deterministic, so a cache filled on one machine (for
+// example while building) serves another with the same toolchain.
+std::string kernel_cache_dir() {
+  const char *env = std::getenv("CSMI_KERNEL_CACHE");
+  if (env) return env;  // empty disables the cache
+  Dl_info info;
+  if (dladdr(reinterpret_cast<void *>(&kernel_cache_dir), &info) && info.dli_fname) {
+    std::string so = info.dli_fname;
+    size_t slash = so.rfind('/');
+    std::string dir = slash == std::string::npos ? "." : so.substr(0, slash);
+    return dir + "/../kcache";
+  }
+  return "";
+}
+
+static std::string cache_path(const std::string &source, const std::vector<std::string> &options) {
+  std::string dir = kernel_cache_dir();
+  if (dir.empty()) return "";
+  std::string k;
+  for (auto &o : options) k += o + "\n";
+  k += "\x01" + source;
+  uint64_t h = 1469598103934665603ull, h2 = 0x9e3779b97f4a7c15ull;
+  for (unsigned char c : k) {
+    h ^= c;
+    h *= 1099511628211ull;
+    h2 = (h2 ^ c) * 0x100000001b3ull + (h2 >> 29);
+  }
+  char name[64];
+  std::snprintf(name, sizeof name, "%016llx%016llx.co", static_cast<unsigned long long>(h),
+                static_cast<unsigned long long>(h2));
+  return dir + "/" + name;
+}
+
+static bool cache_load(const std::string &path, std::vector<char> *code) {
+  if (path.empty()) return false;
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  code->assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  return !code->empty();
+}
+
+static void cache_store(const std::string &path, const std::vector<char> &code) {
+  if (path.empty()) return;
+  std::string dir = path.substr(0, path.rfind('/'));
+  ::mkdir(dir.c_str(), 0755);
+  std::string tmp = path + ".tmp" + std::to_string(::getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) return;
+    f.write(code.data(), static_cast<std::streamsize>(code.size()));
+    if (!f) return;
+  }
+  std::rename(tmp.c_str(), path.c_str());
+}
+
+bool precompile(const std::string &source, const std::vector<std::string> &options, const std::string &arch,
+                std::string *log) {
+  std::vector<std::string> all = base_compile_options(arch);
+  all.insert(all.end(), options.begin(), options.end());
+  std::string path = cache_path(source, all);
+  std::vector<char> code;
+  if (cache_load(path, &code)) return true;
+  if (!rtc_compile(source, all, log, &code)) return false;
+  cache_store(path, code);
+  return true;
+}
+
 bool compile_check(const std::string &source, const std::vector<std::string> &options,
                    const std::string &arch, std::string *log, size_t *code_size) {
   std::vector<std::string> all = base_compile_options(arch);
@@ -237,9 +311,13 @@ hipModule_t build_module(cs_device *device, const std::string &source,
   all.insert(all.end(), options.begin(), options.end());
   std::vector<char> code;
   std::string l;
-  if (!rtc_compile(source, all, &l, &code)) {
-    if (log) *log = l;
-    fail(CS_BUILD_PROGRAM_FAILURE, "hiprtc build failed:\n" + l);
+  std::string path = cache_path(source, all);
+  if (!cache_load(path, &code)) {
+    if (!rtc_compile(source, all, &l, &code)) {
+      if (log) *log = l;
+      fail(CS_BUILD_PROGRAM_FAILURE, "hiprtc build failed:\n" + l);
+    }
+    cache_store(path, code);
   }
   if (log) *log = l;
   set_device(device);

@@ -244,9 +244,69 @@ cs_status cs_fused_cache_clear(cs_device *device);
 cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32_t *outputs,
                                    int noutputs, int out_rank, const int32_t *out_shape,
                                    const cs_fused_options *options, char **source, uint64_t *key);
+/* Many expressions at once: one hiprtc module holds every kernel not already cached (one
+ * hiprtc call instead of n), and out[i] receives the handle of descs[i]. */
+typedef struct cs_fused_desc {
+  const cs_node *nodes;
+  int nnodes;
+  const int32_t *outputs;
+  int noutputs;
+  int out_rank;
+  const int32_t *out_shape;
+  cs_fused_options options;
+} cs_fused_desc;
+cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out);
+/* cs_fused_launch with the workgroup size chosen at launch (0 = the compiled default):
+ * the generated code reads the workgroup size at run time, so one compile serves every
+ * size the autotuner tries. */
+cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
+                                void *out, int block);
+/* Compiles the fused kernel into the on-disk code-object cache without a device, so that a
+ * later cs_fused_compile of the same expression and options only loads it. The cache lives
+ * in $CSMI_KERNEL_CACHE or in `kcache` beside the library's directory. */
+cs_status cs_fused_precompile(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutputs,
+                              int out_rank, const int32_t *out_shape, const cs_fused_options *options);
 /* Compiles `source` for gfx950 with hiprtc without loading it (no device needed); on
  * failure returns CS_BUILD_PROGRAM_FAILURE with the log in *log (free with cs_free). */
 cs_status cs_program_check(const char *source, const char *options, char **log);
+
+/* ---------------------------------------------------------------- batches
+ * Many fused expressions in one persistent kernel. The reference launches one kernel per
+ * expression (Tensors.scala:1291-1392, at most 5 in flight per device, OpenCL.scala:1313);
+ * here every expression's body becomes a device function (with its own vector width and
+ * vectors per thread), and a fixed grid drains a queue of (expression, tile) work items
+ * through an atomic counter. cs_batch_launch enqueues a counter reset and the persistent
+ * kernel, so it can be captured in a graph and replayed. */
+typedef struct cs_batch cs_batch;
+cs_status cs_batch_create(cs_device *device, cs_fused *const *exprs, int n, int block, cs_batch **batch);
+/* args: for each expression in order, its leaf pointers then its output pointer. Copies the
+ * table to the device and waits (call outside graph capture). */
+cs_status cs_batch_bind(cs_stream *stream, cs_batch *batch, const void *const *args, int nargs);
+cs_status cs_batch_launch(cs_stream *stream, cs_batch *batch);
+cs_status cs_batch_info(cs_batch *batch, int64_t *items, int *grid, int *block);
+cs_status cs_batch_release(cs_batch *batch);
+
+/* ---------------------------------------------------------------- cost-model network
+ * A fully connected network widths[0] -> ... -> widths[nw-1] with ReLU after every layer but
+ * the last, trained and evaluated on the device in FP32 (the learned cost model that
+ * predicts fused-kernel times; the reference has no tuning). Parameters are one flat array
+ * [W1 (w0 x w1, row-major [in][out]), b1, W2, b2, ...], initialised He-uniform from `seed`
+ * with zero biases. X, y, target, loss and grads are device pointers; every reduction runs
+ * in a fixed order, so results are deterministic. */
+typedef struct cs_mlp cs_mlp;
+cs_status cs_mlp_create(cs_device *device, const int32_t *widths, int nw, uint32_t seed, cs_mlp **mlp);
+cs_status cs_mlp_param_count(cs_mlp *mlp, int64_t *count);
+cs_status cs_mlp_get_params(cs_stream *stream, cs_mlp *mlp, float *host, int64_t count);
+cs_status cs_mlp_set_params(cs_stream *stream, cs_mlp *mlp, const float *host, int64_t count);
+/* y[n x widths[nw-1]] = network(X[n x widths[0]]) */
+cs_status cs_mlp_forward(cs_stream *stream, cs_mlp *mlp, const float *X, int64_t n, float *y);
+/* Mean squared error of network(X) (one output) against target[n]: *loss (nullable) and the
+ * gradient of every parameter in grads[param_count]. */
+cs_status cs_mlp_backward(cs_stream *stream, cs_mlp *mlp, const float *X, const float *target, int64_t n,
+                          float *loss, float *grads);
+/* One Adam step (beta1 0.9, beta2 0.999, eps 1e-8); t counts steps from 1. */
+cs_status cs_mlp_adam(cs_stream *stream, cs_mlp *mlp, const float *grads, float lr, int t);
+cs_status cs_mlp_release(cs_mlp *mlp);
 
 /* ---------------------------------------------------------------- reductions
  * Replaces `sum` (Tensors.scala:673-771) with a correct tree (the reference's parallel

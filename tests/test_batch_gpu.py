@@ -1,0 +1,72 @@
+"""The persistent batch of uneven expressions: bit-identical to per-expression launches,
+within the propagated error bound of the oracle, and correct on every graph replay."""
+import numpy as np
+import pytest
+
+from compute import batch as bm
+from compute import workloads
+from oracle import fp64
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small_set(gpu_tensors):
+    T = gpu_tensors
+    exprs = workloads.uneven_dags(T, 48, seed=77, max_elements=200_000)
+    eset = bm.ExpressionSet(T, exprs)
+    yield eset
+    eset.release()
+
+
+def test_batch_matches_per_expression_launches(small_set):
+    eset = small_set
+    T = eset.T
+    rng = np.random.default_rng(0)
+    cfgs = [{"vector_width": int(rng.choice([1, 2, 4])), "items_per_thread": int(rng.choice([1, 2, 4])),
+             "block_size": 256} for _ in eset.exprs]
+    fused = bm.compile_many(T, eset.exprs, cfgs)
+    for i, f in enumerate(fused):
+        eset.launch(i, f, int(rng.choice([64, 256, 1024])))
+    each = eset.outputs()
+    eset.clear_outputs()
+    b = bm.Batch(eset, fused, block=256)
+    g = bm.Graph(T, b.launch)
+    for rep in range(3):
+        eset.clear_outputs()
+        g.launch()
+        T.synchronize()
+        got = eset.outputs()
+        for i in range(len(each)):
+            assert np.array_equal(got[i].view(np.uint32), each[i].view(np.uint32)), (rep, i)
+    g.release()
+    b.release()
+    from compute import _capi as capi
+    for f in fused:
+        capi.lib.cs_fused_release(f)
+    # and the results are right: within the propagated bound of the FP32 program
+    for i in range(0, len(each), 6):
+        e = eset.exprs[i]
+        value, bound = fp64.evaluate_with_bound(e.tensor.closure, e.shape, e.leaf_data, ulps=2.0)
+        assert fp64.within_bound(each[i], value, bound).all(), i
+
+
+def test_batch_of_one_and_tiny(gpu_tensors):
+    T = gpu_tensors
+    exprs = workloads.uneven_dags(T, 3, seed=5, min_elements=1, max_elements=70)
+    eset = bm.ExpressionSet(T, exprs)
+    try:
+        fused = bm.compile_many(T, exprs, [bm.DEFAULT] * len(exprs))
+        b = bm.Batch(eset, fused, block=64)
+        b.launch()
+        T.synchronize()
+        got = eset.outputs()
+        for i, e in enumerate(exprs):
+            value, bound = fp64.evaluate_with_bound(e.tensor.closure, e.shape, e.leaf_data, ulps=2.0)
+            assert fp64.within_bound(got[i], value, bound).all()
+        b.release()
+        from compute import _capi as capi
+        for f in fused:
+            capi.lib.cs_fused_release(f)
+    finally:
+        eset.release()
