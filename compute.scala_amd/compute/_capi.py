@@ -212,6 +212,7 @@ _PROTOS = {
     "cs_batch_launch": (I, [P, P]),
     "cs_batch_info": (I, [P, ctypes.POINTER(I64), ctypes.POINTER(I), ctypes.POINTER(I)]),
     "cs_batch_release": (I, [P]),
+    "cs_batch_generate_source": (I, [ctypes.POINTER(CsFusedDesc), I, I, PP, ctypes.POINTER(I64)]),
     "cs_reduce_sum": (I, [P, P, I64, P, P]),
     "cs_reduce_max": (I, [P, P, I64, P]),
     "cs_reduce_rows": (I, [P, P, I64, I64, P, P, P, P]),

@@ -334,14 +334,22 @@ class Emitter {
     return s.str();
   }
 
-  // The same body as a device function over an explicit tile index, for batches.
-  std::string device_function(const std::string &name, int64_t *vectors_out) {
+  // The same body as a device function over an explicit tile index, for batches. It is
+  // not inlined: inlined into one switch, hundreds of bodies share values hoisted across
+  // the cases and the kernel's register count (and occupancy) is set by the sum, not the
+  // largest. Pointers carry the global address space (a function parameter is otherwise
+  // generic and every access a flat one) and the workgroup size is the batch's constant.
+  std::string device_function(const std::string &name, int block, int64_t *vectors_out) {
+    gq_ = "CSMI_G ";
+    bdim_ = std::to_string(block) + "u";
     std::ostringstream s;
-    s << "__device__ __forceinline__ void " << name << "(const unsigned tile, ";
+    s << "__device__ __noinline__ void " << name << "(const unsigned tile, ";
     params(s);
     s << ") {\n";
     body(s, "tile", vectors_out);
     s << "}\n";
+    gq_.clear();
+    bdim_ = "blockDim.x";
     return s.str();
   }
 
@@ -352,6 +360,7 @@ class Emitter {
     p += "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
     p += "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     p += "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
+    p += "#define CSMI_G __attribute__((address_space(1)))\n";
     return p;
   }
 
@@ -368,8 +377,8 @@ class Emitter {
   }
 
   void params(std::ostringstream &s) {
-    for (int i = 0; i < dag_.nleaves; ++i) s << "const float* __restrict__ p" << i << ", ";
-    s << "float* __restrict__ out";
+    for (int i = 0; i < dag_.nleaves; ++i) s << "const " << gq_ << "float* __restrict__ p" << i << ", ";
+    s << gq_ << "float* __restrict__ out";
   }
 
   // tile: the index of the block's slice of blockDim.x * U vectors.
@@ -383,12 +392,12 @@ class Emitter {
     const int nout = static_cast<int>(dag_.outputs.size());
     uniform_.clear();
     fill_of_.clear();
-    s << "  const unsigned vbase = (" << tile << ") * (blockDim.x * " << U << "u) + threadIdx.x;\n";
+    s << "  const unsigned vbase = (" << tile << ") * (" << bdim_ << " * " << U << "u) + threadIdx.x;\n";
     // phase 0: vector index, validity and coordinates of element 0 of every vector
     s << "  unsigned q[" << U << "]; bool live[" << U << "];\n";
     for (int d = 0; d < r; ++d) s << "  int c" << d << "[" << U << "];\n";
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
-    s << "    q[u] = vbase + u * blockDim.x;\n";
+    s << "    q[u] = vbase + u * " << bdim_ << ";\n";
     s << "    live[u] = q[u] < " << vectors << "u;\n";
     s << "    unsigned rest = q[u] * " << V << "u;\n";
     for (int d = r - 1; d >= 0; --d) {
@@ -411,7 +420,7 @@ class Emitter {
     const int run = V * nout;
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
     s << "    if (!live[u]) continue;\n";
-    s << "    float* __restrict__ dst = out + (unsigned long long)q[u] * " << run << "u;\n";
+    s << "    " << gq_ << "float* __restrict__ dst = out + (unsigned long long)q[u] * " << run << "u;\n";
     std::vector<std::string> vals;
     for (int v = 0; v < V; ++v)
       for (int j = 0; j < nout; ++j) vals.push_back(value_of(dag_.outputs[j], v));
@@ -432,8 +441,8 @@ class Emitter {
   }
 
   std::string store(const std::string &addr, const std::string &val, const std::string &type) {
-    if (o_.nt) return "__builtin_nontemporal_store(" + val + ", (" + type + "*)(" + addr + "));";
-    return "*(" + type + "*)(" + addr + ") = " + val + ";";
+    if (o_.nt) return "__builtin_nontemporal_store(" + val + ", (" + gq_ + type + "*)(" + addr + "));";
+    return "*(" + gq_ + type + "*)(" + addr + ") = " + val + ";";
   }
 
   // extract(fill(x)) stands for x
@@ -651,16 +660,16 @@ class Emitter {
       std::string others;
       for (size_t c = 0; c < conds.size(); ++c) others += (c ? " && " : "") + conds[c];
       s << "      const bool " << g << "_ok = " << others << ";\n";
-      s << "      const float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
+      s << "      const " << gq_ << "float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
       s << "      if (" << g << "_ok && " << yi << " >= 0 && " << yi << " + " << (V - 1) << " < " << p.shape[moving]
         << ") {\n";
       for (int h = 0; h < V; h += 4) {
         if (V - h >= 4) {
-          s << "        const csmi_f4 a" << h << "_ = *(const csmi_f4*)(" << g << "_p + " << h << ");\n";
+          s << "        const csmi_f4 a" << h << "_ = *(const " << gq_ << "csmi_f4*)(" << g << "_p + " << h << ");\n";
           for (int v = 0; v < 4; ++v)
             s << "        " << name << "[u][" << h + v << "] = a" << h << "_[" << v << "];\n";
         } else {
-          s << "        const csmi_f2 a" << h << "_ = *(const csmi_f2*)(" << g << "_p + " << h << ");\n";
+          s << "        const csmi_f2 a" << h << "_ = *(const " << gq_ << "csmi_f2*)(" << g << "_p + " << h << ");\n";
           for (int v = 0; v < 2; ++v)
             s << "        " << name << "[u][" << h + v << "] = a" << h << "_[" << v << "];\n";
         }
@@ -697,6 +706,8 @@ class Emitter {
 
   const Dag &dag_;
   Options o_;
+  std::string gq_;                    // pointer qualifier: "" in kernels, global in batch functions
+  std::string bdim_ = "blockDim.x";  // workgroup size expression
   std::ostringstream *body_ = nullptr;
   std::set<int> uniform_;
   std::map<int, int> fill_of_;
@@ -767,6 +778,82 @@ Dag prepare(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutpu
   return dag;
 }
 
+// One member of a batch: its canonical DAG and resolved launch options.
+struct BatchExpr {
+  const Dag *dag;
+  Options opts;
+};
+
+// The persistent batch kernel's source and its work items: (expression, first tile, tile
+// count, 0) per item, expressions with the most work per tile first.
+struct BatchPlan {
+  std::string source;
+  std::vector<uint32_t> items;
+  int max_args = 1;
+};
+
+BatchPlan plan_batch(const std::vector<BatchExpr> &members, int block, int math_mode) {
+  const int n = static_cast<int>(members.size());
+  BatchPlan plan;
+  std::ostringstream src;
+  src << "// csmi batch of " << n << " expressions\n" << Emitter::prelude(math_mode);
+  std::vector<int64_t> tiles(static_cast<size_t>(n)), tile_bytes(static_cast<size_t>(n));
+  std::vector<double> cost(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) {
+    const Dag &dag = *members[i].dag;
+    const Options &o = members[i].opts;
+    int64_t vectors = 0;
+    src << Emitter(dag, o).device_function("expr_" + std::to_string(i), block, &vectors);
+    int64_t per = int64_t(block) * o.U;
+    tiles[i] = (vectors + per - 1) / per;
+    tile_bytes[i] = per * o.V * int64_t(dag.outputs.size() + dag.nleaves) * 4;
+    cost[i] = double(vectors) * o.V * (dag.nleaves + 1 + double(dag.nodes.size()) / 4.0);
+    plan.max_args = std::max(plan.max_args, dag.nleaves + 1);
+  }
+  src << "extern \"C\" __global__ void batch_reset(unsigned* counter) { *counter = 0u; }\n";
+  // Work item = (expression, first tile, tile count). Thread 0 takes the next item from
+  // the counter while the block works on the current one, hiding the atomic's latency.
+  src << "extern \"C\" __global__ __launch_bounds__(" << block << ") void batch_kernel(const uint4* __restrict__ items, "
+         "const unsigned nitems, const float* const* __restrict__ args, unsigned* counter) {\n";
+  src << "  __shared__ unsigned next;\n";
+  src << "  if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n  __syncthreads();\n";
+  // The item index is the same in every lane: readfirstlane makes it scalar, so the item
+  // and its argument pointers come through scalar loads.
+  src << "  unsigned w = __builtin_amdgcn_readfirstlane(next);\n  __syncthreads();\n";
+  src << "  while (w < nitems) {\n";
+  src << "    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n";
+  src << "    const uint4 it = items[w];\n    const float* const* a = args + it.x * " << plan.max_args << "u;\n";
+  src << "    for (unsigned t = it.y; t < it.y + it.z; ++t) {\n";
+  src << "      switch (it.x) {\n";
+  for (int i = 0; i < n; ++i) {
+    const int nl = members[i].dag->nleaves;
+    src << "        case " << i << ": expr_" << i << "(t";
+    for (int k = 0; k < nl; ++k) src << ", (const CSMI_G float*)a[" << k << "]";
+    src << ", (CSMI_G float*)a[" << nl << "]); break;\n";
+  }
+  src << "        default: break;\n      }\n    }\n";
+  src << "    __syncthreads();\n    w = __builtin_amdgcn_readfirstlane(next);\n    __syncthreads();\n  }\n}\n";
+  plan.source = src.str();
+  std::vector<int> order(static_cast<size_t>(n));
+  for (int i = 0; i < n; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
+    return cost[x] / std::max<int64_t>(tiles[x], 1) > cost[y] / std::max<int64_t>(tiles[y], 1);
+  });
+  // Runs of consecutive tiles of about kItemBytes of traffic: few enough items that the
+  // single counter is not a bottleneck, many enough to balance the grid.
+  constexpr int64_t kItemBytes = 256 << 10;
+  for (int i : order) {
+    int64_t run = std::max<int64_t>(1, kItemBytes / std::max<int64_t>(tile_bytes[i], 1));
+    for (int64_t t = 0; t < tiles[i]; t += run) {
+      plan.items.push_back(static_cast<uint32_t>(i));
+      plan.items.push_back(static_cast<uint32_t>(t));
+      plan.items.push_back(static_cast<uint32_t>(std::min(run, tiles[i] - t)));
+      plan.items.push_back(0u);
+    }
+  }
+  return plan;
+}
+
 }  // namespace
 
 extern "C" {
@@ -785,6 +872,33 @@ cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32
     if (!*source) fail(CS_OUT_OF_HOST_MEMORY, "malloc");
     std::memcpy(*source, src.c_str(), src.size() + 1);
     if (key) *key = fnv1a(k);
+  });
+}
+
+cs_status cs_batch_generate_source(const cs_fused_desc *descs, int n, int block, char **source, int64_t *items) {
+  return guard([&] {
+    if (!source) fail(CS_INVALID_VALUE, "source is null");
+    if (n <= 0 || !descs) fail(CS_INVALID_VALUE, "empty batch");
+    if (block < 64 || block > 1024 || block % 64) fail(CS_INVALID_WORK_GROUP_SIZE, "block must be 64..1024");
+    std::vector<Dag> dags(static_cast<size_t>(n));
+    std::vector<BatchExpr> members;
+    for (int i = 0; i < n; ++i) {
+      const cs_fused_desc &d = descs[i];
+      Options o;
+      std::string k;
+      dags[i] = prepare(d.nodes, d.nnodes, d.outputs, d.noutputs, d.out_rank, d.out_shape, &d.options, &o, &k);
+      members.push_back({nullptr, o});
+    }
+    for (int i = 0; i < n; ++i) {
+      members[i].dag = &dags[i];
+      if (members[i].opts.math_mode != members[0].opts.math_mode)
+        fail(CS_INVALID_VALUE, "mixed math modes in one batch");
+    }
+    BatchPlan plan = plan_batch(members, block, members[0].opts.math_mode);
+    *source = static_cast<char *>(std::malloc(plan.source.size() + 1));
+    if (!*source) fail(CS_OUT_OF_HOST_MEMORY, "malloc");
+    std::memcpy(*source, plan.source.c_str(), plan.source.size() + 1);
+    if (items) *items = static_cast<int64_t>(plan.items.size() / 4);
   });
 }
 
@@ -1099,75 +1213,32 @@ cs_status cs_batch_create(cs_device *device, cs_fused *const *exprs, int n, int 
       if (exprs[i]->device != device) fail(CS_INVALID_DEVICE, "expression compiled on another device");
       if (exprs[i]->opts.math_mode != math_mode) fail(CS_INVALID_VALUE, "mixed math modes in one batch");
     }
-    auto *b = new cs_batch();
+    std::vector<BatchExpr> members;
+    for (int i = 0; i < n; ++i) members.push_back({&exprs[i]->dag, exprs[i]->opts});
+    BatchPlan plan = plan_batch(members, block, math_mode);
+    std::unique_ptr<cs_batch, void (*)(cs_batch *)> b(new cs_batch(), [](cs_batch *x) {
+      if (x->module) (void)hipModuleUnload(x->module);
+      (void)hipFree(x->items_dev);
+      (void)hipFree(x->args_dev);
+      (void)hipFree(x->counter);
+      delete x;
+    });
     b->device = device;
     b->block = static_cast<uint32_t>(block);
-    std::ostringstream src;
-    src << "// csmi batch of " << n << " expressions\n" << Emitter::prelude(math_mode);
-    std::vector<int64_t> tiles(static_cast<size_t>(n)), tile_bytes(static_cast<size_t>(n));
-    std::vector<double> cost(static_cast<size_t>(n));
-    int max_args = 1;
-    for (int i = 0; i < n; ++i) {
-      cs_fused *f = exprs[i];
-      int64_t vectors = 0;
-      src << Emitter(f->dag, f->opts).device_function("expr_" + std::to_string(i), &vectors);
-      int64_t per = int64_t(block) * f->opts.U;
-      tiles[i] = (vectors + per - 1) / per;
-      tile_bytes[i] = per * f->opts.V * int64_t(f->dag.outputs.size() + f->nleaves) * 4;
-      cost[i] = double(vectors) * f->opts.V * (f->nleaves + 1 + double(f->dag.nodes.size()) / 4.0);
-      max_args = std::max(max_args, f->nleaves + 1);
-    }
-    b->max_args = max_args;
-    src << "extern \"C\" __global__ void batch_reset(unsigned* counter) { *counter = 0u; }\n";
-    // Work item = (expression, first tile, tile count). Thread 0 takes the next item from
-    // the counter while the block works on the current one, hiding the atomic's latency.
-    src << "extern \"C\" __global__ __launch_bounds__(1024) void batch_kernel(const uint4* __restrict__ items, "
-           "const unsigned nitems, const float* const* __restrict__ args, unsigned* counter) {\n";
-    src << "  __shared__ unsigned next;\n";
-    src << "  if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n  __syncthreads();\n";
-    src << "  unsigned w = next;\n  __syncthreads();\n";
-    src << "  while (w < nitems) {\n";
-    src << "    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n";
-    src << "    const uint4 it = items[w];\n    const float* const* a = args + it.x * " << max_args << "u;\n";
-    src << "    for (unsigned t = it.y; t < it.y + it.z; ++t) {\n";
-    src << "      switch (it.x) {\n";
-    for (int i = 0; i < n; ++i) {
-      src << "        case " << i << ": expr_" << i << "(t";
-      for (int k = 0; k < exprs[i]->nleaves; ++k) src << ", a[" << k << "]";
-      src << ", (float*)a[" << exprs[i]->nleaves << "]); break;\n";
-    }
-    src << "        default: break;\n      }\n    }\n";
-    src << "    __syncthreads();\n    w = next;\n    __syncthreads();\n  }\n}\n";
-    b->source = src.str();
+    b->source = plan.source;
+    b->max_args = plan.max_args;
     std::string log;
+    set_device(device);
     b->module = build_module(device, b->source, math_options(math_mode), &log);
     CS_HIP(hipModuleGetFunction(&b->kernel, b->module, "batch_kernel"));
     CS_HIP(hipModuleGetFunction(&b->reset, b->module, "batch_reset"));
-    // Work items, largest expressions first.
-    std::vector<int> order(static_cast<size_t>(n));
-    for (int i = 0; i < n; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) {
-      return cost[x] / std::max<int64_t>(tiles[x], 1) > cost[y] / std::max<int64_t>(tiles[y], 1);
-    });
-    // Runs of consecutive tiles of about kItemBytes of traffic: few enough items that the
-    // single counter is not a bottleneck, many enough to balance the grid.
-    constexpr int64_t kItemBytes = 256 << 10;
-    std::vector<uint32_t> items;
-    for (int i : order) {
-      int64_t run = std::max<int64_t>(1, kItemBytes / std::max<int64_t>(tile_bytes[i], 1));
-      for (int64_t t = 0; t < tiles[i]; t += run) {
-        items.push_back(static_cast<uint32_t>(i));
-        items.push_back(static_cast<uint32_t>(t));
-        items.push_back(static_cast<uint32_t>(std::min(run, tiles[i] - t)));
-        items.push_back(0u);
-      }
-    }
+    const std::vector<uint32_t> &items = plan.items;
     b->items = static_cast<int64_t>(items.size() / 4);
-    set_device(device);
     CS_HIP(hipMalloc(&b->items_dev, std::max<size_t>(items.size() * 4, 8)));
     if (!items.empty()) CS_HIP(hipMemcpy(b->items_dev, items.data(), items.size() * 4, hipMemcpyHostToDevice));
-    CS_HIP(hipMalloc(&b->args_dev, size_t(n) * max_args * sizeof(void *)));
-    CS_HIP(hipMemset(b->args_dev, 0, size_t(n) * max_args * sizeof(void *)));
+    const size_t table = size_t(n) * size_t(b->max_args) * sizeof(void *);
+    CS_HIP(hipMalloc(&b->args_dev, table));
+    CS_HIP(hipMemset(b->args_dev, 0, table));
     CS_HIP(hipMalloc(reinterpret_cast<void **>(&b->counter), 256));
     CS_HIP(hipMemset(b->counter, 0, 256));
     int occupancy = 0;
@@ -1180,7 +1251,7 @@ cs_status cs_batch_create(cs_device *device, cs_fused *const *exprs, int n, int 
       b->exprs.push_back(exprs[i]);
     }
     for (int i = 0; i < n; ++i) b->total_args += exprs[i]->nleaves + 1;
-    *batch = b;
+    *batch = b.release();
   });
 }
 

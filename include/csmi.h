@@ -285,6 +285,10 @@ cs_status cs_batch_bind(cs_stream *stream, cs_batch *batch, const void *const *a
 cs_status cs_batch_launch(cs_stream *stream, cs_batch *batch);
 cs_status cs_batch_info(cs_batch *batch, int64_t *items, int *grid, int *block);
 cs_status cs_batch_release(cs_batch *batch);
+/* Generates the batch kernel's source without a device (the text cs_batch_create builds):
+ * one device function per expression of `descs` and the persistent kernel. *source is
+ * malloc'ed, to be freed with cs_free; *items (nullable) receives the work-item count. */
+cs_status cs_batch_generate_source(const cs_fused_desc *descs, int n, int block, char **source, int64_t *items);
 
 /* ---------------------------------------------------------------- cost-model network
  * A fully connected network widths[0] -> ... -> widths[nw-1] with ReLU after every layer but
