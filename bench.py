@@ -63,6 +63,29 @@ class Timer(object):
         return ms.value
 
 
+def per_launch_us(capi, stream, launch, count):
+    """Average of `count` launches each bracketed by its own pair of HIP events (eager, no
+    graph): the kernel's own duration, comparable with a profiler's per-dispatch time."""
+    lib = capi.lib
+    evs = []
+    capi.check(lib.cs_stream_synchronize(stream))
+    for i in range(count):
+        e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        capi.check(lib.cs_event_record(stream, ctypes.byref(e0)))
+        launch(i)
+        capi.check(lib.cs_event_record(stream, ctypes.byref(e1)))
+        evs.append((e0, e1))
+    capi.check(lib.cs_stream_synchronize(stream))
+    total = 0.0
+    for e0, e1 in evs:
+        ms = ctypes.c_float()
+        capi.check(lib.cs_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+        total += ms.value
+        lib.cs_event_release(e0)
+        lib.cs_event_release(e1)
+    return total * 1e3 / count
+
+
 def capture(capi, stream, fns):
     lib = capi.lib
     capi.check(lib.cs_graph_begin_capture(stream))
@@ -73,7 +96,7 @@ def capture(capi, stream, fns):
     return g
 
 
-def bench_c2(T, capi, steps, warmup, sets=16, tune=True):
+def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
     from compute import autotune
     from compute import tensors as tz
     lib = capi.lib
@@ -94,7 +117,10 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True):
     out_ptrs = [o.ptr for o in outs]
     tuning = None
     best_cfg = None
-    if tune:
+    if fixed is not None:
+        best_cfg = fixed
+        tuning = {"best": fixed, "fixed": True}
+    elif tune:
         cands = autotune.candidates(256, vector_widths=(4, 8), items=(1, 2, 4), blocks=(256, 512),
                                     nontemporal=(0, 1))
         results = autotune.tune_fused(T, out.closure, out.shape, leaf_sets, out_ptrs, cands, reps=8)
@@ -128,8 +154,10 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True):
     # hot-cache figure: one input set, repeated
     g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
     ms_hot = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_hot)), reps)
+    us_event = per_launch_us(capi, T.stream, lambda i: launch(i % sets), 4 * sets)
     bytes_per = 4 * n + 4 * 256 + 4 * n
     res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
+               us_per_launch_events=us_event,
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
     lib.cs_graph_release(g)
@@ -408,6 +436,11 @@ def main():
     ap.add_argument("--only", default="c2,c1,c3,c4,c5")
     ap.add_argument("--c5-sweep", action="store_true", help="run the exhaustive autotuner and train the cost model")
     ap.add_argument("--out-dir", default=None)
+    ap.add_argument("--c2-config", default=None,
+                    help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
+                         "reference for hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every workload here runs on a single MI355X, "
@@ -417,7 +450,21 @@ def main():
     only = set(args.only.split(","))
     T = Tensors(0)
     log("device: %d CUs" % T.info(capi.INFO_COMPUTE_UNITS))
-    c2 = bench_c2(T, capi, args.steps, args.warmup)
+    if args.copy_mib:
+        nb = args.copy_mib << 20
+        src, dst = T._alloc(nb), T._alloc(nb)
+        capi.check(capi.lib.cs_fill_uniform(T.stream, src.ptr, nb // 4, 9, -1.0, 1.0))
+        for _ in range(8):
+            capi.check(capi.lib.cs_memcpy_dtod(T.stream, dst.ptr, src.ptr, nb))
+        T.synchronize()
+        src.release()
+        dst.release()
+        log("copied %d MiB 8 times" % args.copy_mib)
+    fixed = None
+    if args.c2_config:
+        v, u, b, nt = (int(x) for x in args.c2_config.split(","))
+        fixed = {"vector_width": v, "items_per_thread": u, "block_size": b, "nontemporal_store": nt}
+    c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed)
     log("c2: %.1f GB/s (%.2f us/launch), hot cache %.1f GB/s" % (c2["gbs"], c2["ms_per_launch"] * 1e3,
                                                                 c2["gbs_hot_cache"]))
     also = {}

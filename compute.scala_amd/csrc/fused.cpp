@@ -323,7 +323,7 @@ class Emitter {
   Emitter(const Dag &dag, const Options &o) : dag_(dag), o_(o) {}
 
   // The whole kernel source: a stand-alone kernel over blockIdx.x.
-  std::string generate(int64_t *vectors_out, const std::string &name = "jit_kernel", bool with_prelude = true) {
+  std::string generate(int64_t *vectors_out, const std::string &name, bool with_prelude = true) {
     std::ostringstream s;
     header(s, with_prelude);
     s << "extern \"C\" __global__ __launch_bounds__(1024) void " << name << "(";
@@ -356,7 +356,7 @@ class Emitter {
   // Typedefs (and the accurate math functions) the bodies need, once per module.
   static std::string prelude(int math_mode) {
     std::string p;
-    if (math_mode == CS_MATH_ACCURATE) p += std::string(kPreciseMathSource) + "\n";
+    p += std::string(kPreciseMathSource) + "\n";
     p += "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
     p += "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     p += "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
@@ -496,7 +496,7 @@ class Emitter {
       case CS_OP_NEG: e = "-" + a; break;
       case CS_OP_ABS: e = "fabsf(" + a + ")"; break;
       case CS_OP_SQRT: e = "sqrtf(" + a + ")"; break;
-      case CS_OP_TANH: e = (acc ? "csmi_tanhf(" : "tanhf(") + a + ")"; break;
+      case CS_OP_TANH: e = (acc ? "csmi_tanhf(" : "csmi_tanhf_p(") + a + ")"; break;
       case CS_OP_EXP: e = (fast ? "__expf(" : acc ? "csmi_expf(" : "expf(") + a + ")"; break;
       case CS_OP_LOG: e = (fast ? "__logf(" : "logf(") + a + ")"; break;
       case CS_OP_ADD: e = a + " + " + b; break;
@@ -544,6 +544,8 @@ class Emitter {
       std::string expr;         // index of element 0 of the vector
       int64_t inner_step = 0;   // integer coefficient of the innermost output coordinate
       bool upper_only = false;  // bare parameter: only idx < dim is checked
+      int64_t shift = 0;        // integral constant term
+      bool others_by4 = true;   // integral coefficients of the other coordinates, all multiples of 4
     };
     std::vector<Row> rows(static_cast<size_t>(src_rank));
     if (!ar.has_view) {
@@ -574,8 +576,10 @@ class Emitter {
             std::string cx = "c" + std::to_string(x) + "[u]";
             terms.push_back(m == 1.0 ? cx : cx + " * " + lit);
             if (x == r - 1 && lit.find('.') == std::string::npos) row.inner_step = std::stoll(lit);
+            if (x != r - 1 && (lit.find('.') != std::string::npos || std::stoll(lit) % 4 != 0)) row.others_by4 = false;
           } else {
             terms.push_back(lit);
+            if (lit.find('.') == std::string::npos) row.shift = std::stoll(lit);
           }
         }
         if (terms.empty()) continue;
@@ -661,6 +665,33 @@ class Emitter {
       for (size_t c = 0; c < conds.size(); ++c) others += (c ? " && " : "") + conds[c];
       s << "      const bool " << g << "_ok = " << others << ";\n";
       s << "      const " << gq_ << "float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
+      // A row read shifted by a constant along the innermost dimension (a translate): element 0
+      // of a vector sits at a fixed phase (shift mod 4) inside a 16-byte chunk of the source
+      // row, so the V elements come from V/4 (+1) aligned chunks, each entirely inside or
+      // outside the row (the row length is a multiple of 4). Out-of-row chunks load from the
+      // leaf's base instead and are replaced by the padding: no per-element branches.
+      const Row &mv = rows[moving];
+      if (mv.shift != 0 && V % 4 == 0 && p.shape[moving] % 4 == 0 && mv.others_by4 && !mv.upper_only) {
+        const int phase = static_cast<int>(((mv.shift % 4) + 4) % 4);
+        const int nch = V / 4 + (phase ? 1 : 0);
+        for (int q = 0; q < nch; ++q) {
+          const std::string cs = g + "_s" + std::to_string(q), ck = g + "_k" + std::to_string(q);
+          const std::string w = g + "_w" + std::to_string(q);
+          const int off = 4 * q - phase;
+          s << "      const int " << cs << " = " << yi << " + " << off << ";\n";
+          s << "      const bool " << ck << " = " << g << "_ok && " << cs << " >= 0 && " << cs << " < "
+            << p.shape[moving] << ";\n";
+          s << "      const csmi_f4 " << w << "l = *(const " << gq_ << "csmi_f4*)(" << ck << " ? " << g << "_p + "
+            << off << " : " << ptr << ");\n";
+          s << "      const csmi_f4 " << w << " = " << ck << " ? " << w << "l : csmi_f4{" << pad << ", " << pad << ", "
+            << pad << ", " << pad << "};\n";
+        }
+        for (int v = 0; v < V; ++v)
+          s << "      " << name << "[u][" << v << "] = " << g << "_w" << (phase + v) / 4 << "[" << (phase + v) % 4
+            << "];\n";
+        s << "  }\n";
+        return;
+      }
       s << "      if (" << g << "_ok && " << yi << " >= 0 && " << yi << " + " << (V - 1) << " < " << p.shape[moving]
         << ") {\n";
       for (int h = 0; h < V; h += 4) {
@@ -770,6 +801,14 @@ struct cs_batch {
 
 namespace {
 
+// Kernel symbol of a compiled expression: its structural hash and launch shape, so that
+// profiles tell expressions and configurations apart.
+std::string kernel_name(const std::string &key, const Options &o) {
+  char buf[64];
+  std::snprintf(buf, sizeof buf, "csmi_fused_%016llx_v%du%d", static_cast<unsigned long long>(fnv1a(key)), o.V, o.U);
+  return buf;
+}
+
 Dag prepare(const cs_node *nodes, int nnodes, const int32_t *outputs, int noutputs, int out_rank,
             const int32_t *out_shape, const cs_fused_options *options, Options *o, std::string *key) {
   Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
@@ -867,7 +906,7 @@ cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32
     std::string k;
     Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &k);
     int64_t vectors = 0;
-    std::string src = Emitter(dag, o).generate(&vectors);
+    std::string src = Emitter(dag, o).generate(&vectors, kernel_name(k, o));
     *source = static_cast<char *>(std::malloc(src.size() + 1));
     if (!*source) fail(CS_OUT_OF_HOST_MEMORY, "malloc");
     std::memcpy(*source, src.c_str(), src.size() + 1);
@@ -909,7 +948,7 @@ cs_status cs_fused_precompile(const cs_node *nodes, int nnodes, const int32_t *o
     std::string k;
     Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &k);
     int64_t vectors = 0;
-    std::string src = Emitter(dag, o).generate(&vectors);
+    std::string src = Emitter(dag, o).generate(&vectors, kernel_name(k, o));
     std::string log;
     if (!precompile(src, math_options(o.math_mode), "gfx950", &log))
       fail(CS_BUILD_PROGRAM_FAILURE, "hiprtc build failed:\n" + log);
@@ -958,7 +997,8 @@ cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
       }
     }
     int64_t vectors = 0;
-    std::string src = Emitter(dag, o).generate(&vectors);
+    const std::string name = kernel_name(key, o);
+    std::string src = Emitter(dag, o).generate(&vectors, name);
     std::string log;
     auto mod = std::make_shared<ModuleRef>();
     mod->ordinal = device->ordinal;
@@ -969,7 +1009,7 @@ cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
     f->hash = fnv1a(key);
     f->source = src;
     f->module = mod;
-    CS_HIP(hipModuleGetFunction(&f->function, mod->module, "jit_kernel"));
+    CS_HIP(hipModuleGetFunction(&f->function, mod->module, name.c_str()));
     f->nleaves = dag.nleaves;
     int64_t total = 1;
     for (auto d : dag.out_shape) total *= d;
@@ -1030,7 +1070,7 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
       std::vector<int64_t> vectors(group.second.size());
       for (size_t k = 0; k < group.second.size(); ++k) {
         int i = group.second[k];
-        src << Emitter(dags[i], opts[i]).generate(&vectors[k], "jit_kernel_" + std::to_string(k), false);
+        src << Emitter(dags[i], opts[i]).generate(&vectors[k], kernel_name(keys[i], opts[i]), false);
       }
       std::string log;
       auto mod = std::make_shared<ModuleRef>();
@@ -1043,9 +1083,10 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
         f->key = keys[i];
         f->hash = fnv1a(keys[i]);
         int64_t v = 0;
-        f->source = Emitter(dags[i], opts[i]).generate(&v);
+        const std::string name = kernel_name(keys[i], opts[i]);
+        f->source = Emitter(dags[i], opts[i]).generate(&v, name);
         f->module = mod;
-        CS_HIP(hipModuleGetFunction(&f->function, mod->module, ("jit_kernel_" + std::to_string(k)).c_str()));
+        CS_HIP(hipModuleGetFunction(&f->function, mod->module, name.c_str()));
         f->nleaves = dags[i].nleaves;
         int64_t total = 1;
         for (auto d : dags[i].out_shape) total *= d;

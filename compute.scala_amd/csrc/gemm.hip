@@ -13,16 +13,23 @@
 //                   each row of A and each column of B (amax / 448 rounded up to 2^e),
 //                   applied by v_mfma_scale_f32_32x32x64_f8f6f4, FP32 accumulate
 //
-// Structure: a pre-pass converts A to a K-contiguous operand X[Mp][Kp] and B to its
-// transpose Y[Np][Kp] (zero padded to the tile sizes, split or quantised as the variant
-// needs) in the stream's scratch; the main kernel computes C[m][n] = sum_k X[m][k] Y[n][k].
-// It runs 256 threads (4 wave64s in 2x2) per 256x256 output tile; each wave owns 128x128
-// as 4x4 MFMA tiles of 32x32 (256 FP32 accumulators per lane). K advances in stages staged
-// through LDS (rows padded by 16 bytes against bank conflicts); the next stage's global
-// loads are in flight while the MFMAs of the current one run.
+// A pre-pass converts A to a K-contiguous operand X[Mp][.] and B to its transpose Y[Np][.]
+// (zero padded to the tile sizes; split, interleaved or quantised as the variant needs) in
+// the stream's scratch; the main kernel computes C[m][n] = sum_k X[m][k] Y[n][k] with 256
+// threads (4 wave64s in 2x2) per 256x256 output tile, each wave owning 128x128 as 4x4 MFMA
+// tiles of 32x32 (256 FP32 accumulators per lane).
+//
+// BF16, BF16X3 and FP8 share one pipeline. A K stage is 128 bytes of every operand row
+// (64 BF16 values, 32 hi/lo pairs or 128 e4m3 values); two stages live in LDS (128 KiB).
+// The next stage streams global -> LDS directly (buffer_load ... lds, no registers) while
+// the MFMAs of the current one run; inside a stage the fragments of the next MFMA step are
+// read from LDS while the current step's MFMAs issue. Each 128-byte LDS row stores its
+// 16-byte piece p at slot p ^ (row & 7), so the fragment reads of 8 consecutive rows hit
+// 8 different bank groups. FP32 keeps a register-staged single-buffer pipeline.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "internal.h"
 
@@ -33,31 +40,28 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) void lds_void;
 
 enum Kind { kF32 = 0, kBF16 = 1, kBF16X3 = 2, kFP8 = 3 };
-
-template <int KIND>
-struct Traits;
-template <>
-struct Traits<kF32> {
-  static constexpr int kElem = 4, kStageK = 16, kInstrK = 2, kParts = 1;
-};
-template <>
-struct Traits<kBF16> {
-  static constexpr int kElem = 2, kStageK = 32, kInstrK = 16, kParts = 1;
-};
-template <>
-struct Traits<kBF16X3> {
-  static constexpr int kElem = 2, kStageK = 32, kInstrK = 16, kParts = 2;
-};
-template <>
-struct Traits<kFP8> {
-  static constexpr int kElem = 1, kStageK = 128, kInstrK = 64, kParts = 1;
-};
 
 constexpr int kTile = 256;     // output tile edge (rows and columns)
 constexpr int kThreads = 256;  // 4 waves
 constexpr int kWaveTiles = 4;  // 4x4 MFMA tiles of 32x32 per wave
+
+// Blocks are dispatched round-robin over the 8 XCDs, each with its own L2. The tile of
+// block b: the XCDs get contiguous ranges of tiles, walked in groups of 4 tile rows (column
+// by column inside a group), so one XCD's blocks share 4 A panels and a run of B panels.
+__device__ __forceinline__ void tile_coords(int b, int tiles_m, int tiles_n, int &tm, int &tn) {
+  const int total = tiles_m * tiles_n;
+  int t = b;
+  if (total % 8 == 0) t = (b % 8) * (total / 8) + b / 8;
+  constexpr int G = 4;
+  const int per_group = G * tiles_n;
+  const int g = t / per_group, local = t % per_group;
+  const int rows = min(G, tiles_m - g * G);
+  tm = g * G + local % rows;
+  tn = local / rows;
+}
 
 // ---------------------------------------------------------------------------- pre-pass
 __device__ __forceinline__ uint16_t bf16_rne(float x) {
@@ -68,43 +72,86 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// Converts rows of a row-major matrix (rows x cols, leading dimension ld) into a padded
-// [rows_p][cols_p] operand. kTranspose reads the source as its transpose (B -> Y).
-// parts == 2 writes the BF16 high part to out0 and the low part to out1.
-template <int KIND, bool kTranspose>
-__global__ __launch_bounds__(256) void convert_kernel(const float *__restrict__ src, int rows, int cols, int ld,
-                                                      int rows_p, int cols_p, void *out0, void *out1) {
+// FP32 copy of a row-major matrix (rows x cols, leading dimension ld) into a zero-padded
+// [rows_p][cols_p] operand; kTranspose reads the source as its transpose (B -> Y).
+template <bool kTranspose>
+__global__ __launch_bounds__(256) void convert_f32_kernel(const float *__restrict__ src, int rows, int cols, int ld,
+                                                          int rows_p, int cols_p, float *__restrict__ out) {
   __shared__ float tile[32][33];
   const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
   const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
   for (int i = ty; i < 32; i += 8) {
-    int r = r0 + i, c = c0 + tx;
     float v = 0.0f;
-    if (kTranspose) {  // output row r = source column r; output col c = source row c
-      if (r < rows && c < cols) v = src[int64_t(c) * ld + r];
+    if (kTranspose) {  // tile[i][tx] = source (c0 + i, r0 + tx), stored at output (r0 + tx, c0 + i)
+      const int sr = c0 + i, sc = r0 + tx;
+      if (sr < cols && sc < rows) v = src[int64_t(sr) * ld + sc];
     } else {
+      const int r = r0 + i, c = c0 + tx;
       if (r < rows && c < cols) v = src[int64_t(r) * ld + c];
     }
     tile[i][tx] = v;
   }
   __syncthreads();
   for (int i = ty; i < 32; i += 8) {
-    int r = r0 + i, c = c0 + tx;
+    const int r = r0 + i, c = c0 + tx;
     if (r >= rows_p || c >= cols_p) continue;
-    float v = tile[i][tx];
-    int64_t o = int64_t(r) * cols_p + c;
-    if (KIND == kF32) {
-      static_cast<float *>(out0)[o] = v;
-    } else {
-      uint16_t h = bf16_rne(v);
-      static_cast<uint16_t *>(out0)[o] = h;
-      if (KIND == kBF16X3) static_cast<uint16_t *>(out1)[o] = bf16_rne(v - bf16_to_f32(h));
-    }
+    out[int64_t(r) * cols_p + c] = kTranspose ? tile[tx][i] : tile[i][tx];
   }
 }
 
-// FP8 quantisation of a [rows][cols] FP32 matrix (already K-contiguous, zero padded) in
-// blocks of 128 along each row: one wave per block, scale 2^e with 2^e >= amax / 448.
+// BF16 operand rows for the LDS pipeline: output row r holds source row r (kTranspose: source
+// column r) along K. kSplit == false: Kp BF16 values per row. kSplit == true: per 32-wide K
+// block, 32 BF16 high parts then 32 BF16 low parts (128 bytes), so that one K stage of a row
+// is 128 contiguous bytes in every variant. A block covers 32 rows x 64 K values; each
+// thread writes 8 consecutive K values.
+template <bool kTranspose, bool kSplit>
+__global__ __launch_bounds__(256) void pack_bf16_kernel(const float *__restrict__ src, int rows, int cols, int ld,
+                                                        int rows_p, int kp, char *__restrict__ out) {
+  __shared__ float tile[64][33];  // [k][row]
+  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 64;
+  const int t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    float v = 0.0f;
+    if (kTranspose) {  // source row k (contiguous along r)
+      const int r = t % 32, k = t / 32 + 8 * i;
+      if (k0 + k < cols && r0 + r < rows) v = src[int64_t(k0 + k) * ld + (r0 + r)];
+      tile[k][r] = v;
+    } else {  // source row r (contiguous along k)
+      const int k = t % 64, r = t / 64 + 4 * i;
+      if (r0 + r < rows && k0 + k < cols) v = src[int64_t(r0 + r) * ld + (k0 + k)];
+      tile[k][r] = v;
+    }
+  }
+  __syncthreads();
+  const int r = t / 8, c = t % 8;
+  const int k = k0 + 8 * c;
+  if (r0 + r >= rows_p || k >= kp) return;
+  uint16_t hi[8], lo[8];
+#pragma unroll
+  for (int e = 0; e < 8; ++e) {
+    const float v = tile[8 * c + e][r];
+    hi[e] = bf16_rne(v);
+    lo[e] = bf16_rne(v - bf16_to_f32(hi[e]));
+  }
+  auto pack = [](const uint16_t *h) {
+    return i32x4{int(h[0] | (uint32_t(h[1]) << 16)), int(h[2] | (uint32_t(h[3]) << 16)),
+                 int(h[4] | (uint32_t(h[5]) << 16)), int(h[6] | (uint32_t(h[7]) << 16))};
+  };
+  if (kSplit) {
+    char *row = out + int64_t(r0 + r) * (int64_t(kp) * 4);
+    char *p = row + (k / 32) * 128 + (k % 32) * 2;
+    *reinterpret_cast<i32x4 *>(p) = pack(hi);
+    *reinterpret_cast<i32x4 *>(p + 64) = pack(lo);
+  } else {
+    char *row = out + int64_t(r0 + r) * (int64_t(kp) * 2);
+    *reinterpret_cast<i32x4 *>(row + int64_t(k) * 2) = pack(hi);
+  }
+}
+
+// FP8 quantisation of a [rows][cols] FP32 matrix (K-contiguous, zero padded) in blocks of
+// 128 along each row: one wave per block, scale 2^e with 2^e >= amax / 448. Scales are
+// written K-block-major, scales[kb * rows + r], so that a wave reads 32 consecutive bytes.
 __global__ __launch_bounds__(256) void quantize_fp8_kernel(const float *__restrict__ src, int rows, int cols,
                                                            uint8_t *__restrict__ q, uint8_t *__restrict__ scales) {
   const int wave = (blockIdx.x * 256 + threadIdx.x) / 64;
@@ -131,125 +178,50 @@ __global__ __launch_bounds__(256) void quantize_fp8_kernel(const float *__restri
   int packed1 = __builtin_amdgcn_cvt_pk_fp8_f32(x1 * inv, 0.0f, 0, false);
   qp[lane] = static_cast<uint8_t>(packed0 & 0xff);
   qp[lane + 64] = static_cast<uint8_t>(packed1 & 0xff);
-  if (lane == 0) scales[int64_t(r) * blocks_per_row + kb] = static_cast<uint8_t>(e + 127);
+  if (lane == 0) scales[int64_t(kb) * rows + r] = static_cast<uint8_t>(e + 127);
 }
 
-// ---------------------------------------------------------------------------- main kernel
-template <int KIND>
-struct Frag;
-template <>
-struct Frag<kF32> {
-  float v;
-};
-template <>
-struct Frag<kBF16> {
-  bf16x8 v;
-};
-template <>
-struct Frag<kBF16X3> {
-  bf16x8 v;
-};
-template <>
-struct Frag<kFP8> {
-  i32x8 v;
-};
+// ---------------------------------------------------------------------------- FP32 kernel
+constexpr int kF32StageK = 16;                      // K values per stage
+constexpr int kF32RowBytes = kF32StageK * 4;        // 64
+constexpr int kF32Stride = kF32RowBytes + 16;       // padded LDS row
+constexpr int kF32Chunks = kTile * kF32RowBytes / 16 / kThreads;  // 16-byte loads per thread per operand
 
-template <int KIND>
-__device__ __forceinline__ void mfma(const Frag<KIND> &a, const Frag<KIND> &b, f32x16 &c, int sa, int sb);
-template <>
-__device__ __forceinline__ void mfma<kF32>(const Frag<kF32> &a, const Frag<kF32> &b, f32x16 &c, int, int) {
-  c = __builtin_amdgcn_mfma_f32_32x32x2f32(a.v, b.v, c, 0, 0, 0);
-}
-template <>
-__device__ __forceinline__ void mfma<kBF16>(const Frag<kBF16> &a, const Frag<kBF16> &b, f32x16 &c, int, int) {
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, c, 0, 0, 0);
-}
-template <>
-__device__ __forceinline__ void mfma<kBF16X3>(const Frag<kBF16X3> &a, const Frag<kBF16X3> &b, f32x16 &c, int,
-                                              int) {
-  c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a.v, b.v, c, 0, 0, 0);
-}
-template <>
-__device__ __forceinline__ void mfma<kFP8>(const Frag<kFP8> &a, const Frag<kFP8> &b, f32x16 &c, int sa, int sb) {
-  // cbsz = blgp = 0: both operands OCP e4m3; one E8M0 scale per lane (byte 0)
-  c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a.v, b.v, c, 0, 0, 0, sa, 0, sb);
-}
-
-template <int KIND>
-__device__ __forceinline__ Frag<KIND> lds_frag(const char *p);
-template <>
-__device__ __forceinline__ Frag<kF32> lds_frag<kF32>(const char *p) {
-  return Frag<kF32>{*reinterpret_cast<const float *>(p)};
-}
-template <>
-__device__ __forceinline__ Frag<kBF16> lds_frag<kBF16>(const char *p) {
-  return Frag<kBF16>{__builtin_bit_cast(bf16x8, *reinterpret_cast<const i32x4 *>(p))};
-}
-template <>
-__device__ __forceinline__ Frag<kBF16X3> lds_frag<kBF16X3>(const char *p) {
-  return Frag<kBF16X3>{__builtin_bit_cast(bf16x8, *reinterpret_cast<const i32x4 *>(p))};
-}
-template <>
-__device__ __forceinline__ Frag<kFP8> lds_frag<kFP8>(const char *p) {
-  return Frag<kFP8>{*reinterpret_cast<const i32x8 *>(p)};
-}
-
-// X: [Mp][Kp] operand(s), Y: [Np][Kp] operand(s); parts == 2 adds the low halves X1, Y1.
-// sx / sy: FP8 scales [Mp][Kp/128] and [Np][Kp/128].
-template <int KIND>
-__global__ __launch_bounds__(kThreads, 1) void gemm_nt_kernel(int M, int N, int Kp, const char *__restrict__ X0,
-                                                             const char *__restrict__ X1, const char *__restrict__ Y0,
-                                                             const char *__restrict__ Y1, const uint8_t *__restrict__ sx,
-                                                             const uint8_t *__restrict__ sy, float *__restrict__ C,
-                                                             int ldc) {
-  using T = Traits<KIND>;
-  constexpr int kRowBytes = T::kStageK * T::kElem;  // bytes of one operand row per stage
-  constexpr int kStride = kRowBytes + 16;           // padded LDS row
-  constexpr int kInstr = T::kStageK / T::kInstrK;   // MFMAs per tile per stage
-  constexpr int kLaneBytes = kRowBytes / 2 / kInstr;
-  constexpr int kChunks = kTile * kRowBytes / 16 / kThreads;  // 16-byte loads per thread per operand
-  constexpr int kParts = T::kParts;
-  extern __shared__ __attribute__((aligned(16))) char lds[];
-  char *xs = lds;                                // [kParts][kTile][kStride]
-  char *ys = lds + kParts * kTile * kStride;      // [kParts][kTile][kStride]
-
+// X: [Mp][Kp] FP32, Y: [Np][Kp] FP32 (Kp a multiple of 16).
+__global__ __launch_bounds__(kThreads, 1) void gemm_f32_kernel(int M, int N, int Kp, int tiles_m, int tiles_n,
+                                                               const float *__restrict__ X,
+                                                               const float *__restrict__ Y, float *__restrict__ C,
+                                                               int ldc) {
+  __shared__ __attribute__((aligned(16))) char lds[2 * kTile * kF32Stride];
+  char *xs = lds, *ys = lds + kTile * kF32Stride;
   const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int wm = wave / 2, wn = wave % 2;
   const int half = lane / 32, l32 = lane % 32;
-  // Tile order: consecutive blocks walk down a column of tiles so that blocks on the same
-  // XCD (blockIdx % 8) share B panels in their L2.
-  const int tiles_m = (M + kTile - 1) / kTile;
-  const int m0 = (blockIdx.x % tiles_m) * kTile;
-  const int n0 = (blockIdx.x / tiles_m) * kTile;
-  const int64_t kpb = int64_t(Kp) * T::kElem;  // operand row bytes in global memory
-
-  i32x4 rx[kParts][kChunks], ry[kParts][kChunks];
+  int tm, tn;
+  tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
+  const int m0 = tm * kTile, n0 = tn * kTile;
+  const char *xg = reinterpret_cast<const char *>(X), *yg = reinterpret_cast<const char *>(Y);
+  const int64_t kpb = int64_t(Kp) * 4;
+  i32x4 rx[kF32Chunks], ry[kF32Chunks];
   auto load_stage = [&](int ks) {
 #pragma unroll
-    for (int c = 0; c < kChunks; ++c) {
+    for (int c = 0; c < kF32Chunks; ++c) {
       const int idx = threadIdx.x + c * kThreads;
-      const int row = idx / (kRowBytes / 16), col = idx % (kRowBytes / 16);
-      const int64_t go = int64_t(ks) * kRowBytes + col * 16;
-      rx[0][c] = *reinterpret_cast<const i32x4 *>(X0 + (m0 + row) * kpb + go);
-      ry[0][c] = *reinterpret_cast<const i32x4 *>(Y0 + (n0 + row) * kpb + go);
-      if (kParts == 2) {
-        rx[kParts - 1][c] = *reinterpret_cast<const i32x4 *>(X1 + (m0 + row) * kpb + go);
-        ry[kParts - 1][c] = *reinterpret_cast<const i32x4 *>(Y1 + (n0 + row) * kpb + go);
-      }
+      const int row = idx / (kF32RowBytes / 16), col = idx % (kF32RowBytes / 16);
+      const int64_t go = int64_t(ks) * kF32RowBytes + col * 16;
+      rx[c] = *reinterpret_cast<const i32x4 *>(xg + (m0 + row) * kpb + go);
+      ry[c] = *reinterpret_cast<const i32x4 *>(yg + (n0 + row) * kpb + go);
     }
   };
   auto store_stage = [&]() {
 #pragma unroll
-    for (int p = 0; p < kParts; ++p)
-#pragma unroll
-      for (int c = 0; c < kChunks; ++c) {
-        const int idx = threadIdx.x + c * kThreads;
-        const int row = idx / (kRowBytes / 16), col = idx % (kRowBytes / 16);
-        *reinterpret_cast<i32x4 *>(xs + (p * kTile + row) * kStride + col * 16) = rx[p][c];
-        *reinterpret_cast<i32x4 *>(ys + (p * kTile + row) * kStride + col * 16) = ry[p][c];
-      }
+    for (int c = 0; c < kF32Chunks; ++c) {
+      const int idx = threadIdx.x + c * kThreads;
+      const int row = idx / (kF32RowBytes / 16), col = idx % (kF32RowBytes / 16);
+      *reinterpret_cast<i32x4 *>(xs + row * kF32Stride + col * 16) = rx[c];
+      *reinterpret_cast<i32x4 *>(ys + row * kF32Stride + col * 16) = ry[c];
+    }
   };
-
   f32x16 acc[kWaveTiles][kWaveTiles];
 #pragma unroll
   for (int i = 0; i < kWaveTiles; ++i)
@@ -257,44 +229,28 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_nt_kernel(int M, int N, int 
     for (int j = 0; j < kWaveTiles; ++j)
 #pragma unroll
       for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-
-  const int stages = Kp / T::kStageK;
+  const int stages = Kp / kF32StageK;
   load_stage(0);
   for (int ks = 0; ks < stages; ++ks) {
     store_stage();
     __syncthreads();
     if (ks + 1 < stages) load_stage(ks + 1);
-    int sa[kWaveTiles], sb[kWaveTiles];
+    // instruction s of the stage: lane half h reads K value 8h + s (a K permutation that
+    // A and B share, so the sum is unchanged)
 #pragma unroll
-    for (int i = 0; i < kWaveTiles; ++i) {
-      sa[i] = sb[i] = 127;
-      if (KIND == kFP8) {
-        const int kb = ks, kbs = Kp / 128;
-        sa[i] = sx[int64_t(m0 + wm * 128 + i * 32 + l32) * kbs + kb];
-        sb[i] = sy[int64_t(n0 + wn * 128 + i * 32 + l32) * kbs + kb];
+    for (int s = 0; s < kF32StageK / 2; ++s) {
+      const int off = half * (kF32RowBytes / 2) + s * 4;
+      float fa[kWaveTiles], fb[kWaveTiles];
+#pragma unroll
+      for (int i = 0; i < kWaveTiles; ++i) {
+        fa[i] = *reinterpret_cast<const float *>(xs + (wm * 128 + i * 32 + l32) * kF32Stride + off);
+        fb[i] = *reinterpret_cast<const float *>(ys + (wn * 128 + i * 32 + l32) * kF32Stride + off);
       }
-    }
-#pragma unroll
-    for (int s = 0; s < kInstr; ++s) {
-      const int off = half * (kRowBytes / 2) + s * kLaneBytes;
-      Frag<KIND> fa[kParts][kWaveTiles], fb[kParts][kWaveTiles];
-#pragma unroll
-      for (int p = 0; p < kParts; ++p)
-#pragma unroll
-        for (int i = 0; i < kWaveTiles; ++i) {
-          fa[p][i] = lds_frag<KIND>(xs + (p * kTile + wm * 128 + i * 32 + l32) * kStride + off);
-          fb[p][i] = lds_frag<KIND>(ys + (p * kTile + wn * 128 + i * 32 + l32) * kStride + off);
-        }
 #pragma unroll
       for (int i = 0; i < kWaveTiles; ++i)
 #pragma unroll
-        for (int j = 0; j < kWaveTiles; ++j) {
-          if (kParts == 2) {  // small products first: Al.Bh + Ah.Bl, then Ah.Bh
-            mfma<KIND>(fa[1][i], fb[0][j], acc[i][j], 0, 0);
-            mfma<KIND>(fa[0][i], fb[1][j], acc[i][j], 0, 0);
-          }
-          mfma<KIND>(fa[0][i], fb[0][j], acc[i][j], sa[i], sb[j]);
-        }
+        for (int j = 0; j < kWaveTiles; ++j)
+          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
     }
     __syncthreads();
   }
@@ -311,64 +267,335 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_nt_kernel(int M, int N, int 
       }
 }
 
-// LDS of one stage: the X and Y panels (kParts each) of kTile padded rows.
+// ------------------------------------------------------------- BF16 / BF16X3 / FP8 kernel
+constexpr int kRowB = 128;                          // bytes of one operand row per K stage
+constexpr int kPanelB = kTile * kRowB;              // 32 KiB: one operand's rows of a stage
+constexpr int kStageB = 2 * kPanelB;                // X and Y panels
+constexpr size_t kLdsBytes = size_t(2) * kStageB;   // two stages in flight: 128 KiB
+
 template <int KIND>
-size_t lds_bytes() {
-  using T = Traits<KIND>;
-  return size_t(2) * T::kParts * kTile * (T::kStageK * T::kElem + 16);
+struct Pipe;
+template <>
+struct Pipe<kBF16> {
+  static constexpr int kBytesPerK = 2, kK = 64, kSteps = 4;  // 4 MFMA K steps of 16
+  typedef bf16x8 Frag;
+};
+template <>
+struct Pipe<kBF16X3> {
+  static constexpr int kBytesPerK = 4, kK = 32, kSteps = 2;  // hi and lo per K value
+  typedef bf16x8 Frag;
+};
+template <>
+struct Pipe<kFP8> {
+  static constexpr int kBytesPerK = 1, kK = 128, kSteps = 2;  // 2 MFMA K steps of 64
+  typedef i32x8 Frag;
+};
+
+#define CSMI_SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+
+template <int KIND>
+__device__ __forceinline__ void mma(const typename Pipe<KIND>::Frag &a, const typename Pipe<KIND>::Frag &b,
+                                    f32x16 &c, int sa, int sb) {
+  if constexpr (KIND == kFP8) {
+    // cbsz = blgp = 0: both operands OCP e4m3; one E8M0 scale per lane (byte 0)
+    c = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, sa, 0, sb);
+  } else {
+    (void)sa;
+    (void)sb;
+    c = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+  }
+}
+
+// Runs stage(ks, more) for ks = 0 .. stages - 1 with `more` (a next stage exists) as a
+// compile-time constant.
+template <class Stage>
+__device__ __forceinline__ void run_stages(int stages, Stage &stage) {
+  for (int ks = 0; ks + 1 < stages; ++ks) stage(ks, std::true_type());
+  stage(stages - 1, std::false_type());
+}
+
+// X: [Mp] rows of Kp * kBytesPerK bytes, Y: [Np] rows likewise. sx / sy: FP8 scales
+// [Kp/128][Mp] and [Kp/128][Np]. Host guarantees 256 * row bytes < 2^31.
+template <int KIND>
+__global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int Kp, int Mp, int Np,
+                                                               const char *__restrict__ X,
+                                                               const char *__restrict__ Y,
+                                                               const uint8_t *__restrict__ sx,
+                                                               const uint8_t *__restrict__ sy,
+                                                               float *__restrict__ C, int ldc) {
+  using P = Pipe<KIND>;
+  using F = typename P::Frag;
+  extern __shared__ __attribute__((aligned(1024))) char lds[];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int wm = wave >> 1, wn = wave & 1, half = lane >> 5, l32 = lane & 31;
+  int tm, tn;
+  tile_coords(blockIdx.x, Mp / kTile, Np / kTile, tm, tn);
+  const int m0 = tm * kTile, n0 = tn * kTile;
+  const int rowb = Kp * P::kBytesPerK;
+  const int stages = Kp / P::kK;
+
+  // Global -> LDS. One DMA instruction of a wave fills a 1 KiB chunk = 8 rows of a panel,
+  // lane l writing LDS bytes 16 l .. 16 l + 15: slot l & 7 of row l >> 3, which holds piece
+  // (l & 7) ^ (l >> 3) of that row. DMA d (0..15) of a stage: operand d & 1, chunk 4 (d >> 1)
+  // + wave, i.e. rows 32 (d >> 1) + 8 wave .. + 7.
+  const int drow = lane >> 3;
+  const unsigned voff = unsigned((drow + 8 * wave) * rowb + (((lane & 7) ^ drow) << 4));
+  const __amdgpu_buffer_rsrc_t rx =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(X) + int64_t(m0) * rowb, 0, kTile * rowb, 0x00020000);
+  const __amdgpu_buffer_rsrc_t ry =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(Y) + int64_t(n0) * rowb, 0, kTile * rowb, 0x00020000);
+  auto dma = [&](int d, int ks, int buf) {
+    const int j = d >> 1;
+    const unsigned soff = unsigned(32 * j * rowb + ks * kRowB);
+    char *l = lds + buf * kStageB + (d & 1) * kPanelB + (4 * j + wave) * 1024;
+    __builtin_amdgcn_raw_ptr_buffer_load_lds((d & 1) ? ry : rx, (lds_void *)(l), 16, voff, soff, 0, 0);
+  };
+
+  // Fragment reads: lane (half, l32) of tile i reads row wm*128 + 32 i + l32 (A) or
+  // wn*128 + 32 i + l32 (B); piece p sits at slot p ^ (row & 7) = p ^ (l32 & 7).
+  const int sw = l32 & 7;
+  const int arow = (wm * 128 + l32) * kRowB, brow = kPanelB + (wn * 128 + l32) * kRowB;
+  auto piece = [&](int buf, int row, int p) -> i32x4 {
+    return *reinterpret_cast<const i32x4 *>(lds + buf * kStageB + row + ((p ^ sw) << 4));
+  };
+  // BF16 step s uses piece 2 s + half (K 16 s + 8 half .. + 7, the MFMA's own K order);
+  // BF16X3 step s: high part piece 2 s + half, low part 4 + 2 s + half; FP8 step s: pieces
+  // 4 s + 2 half and 4 s + 2 half + 1 (K 64 s + 32 half .. + 31).
+  auto frag = [&](int buf, int row, int s, int part) -> F {
+    if constexpr (KIND == kFP8) {
+      const int p0 = 4 * s + 2 * half;
+      const i32x4 x = piece(buf, row, p0), y = piece(buf, row, p0 + 1);
+      return i32x8{x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]};
+    } else {
+      return __builtin_bit_cast(bf16x8, piece(buf, row, 4 * part + 2 * s + half));
+    }
+  };
+
+  f32x16 acc[kWaveTiles][kWaveTiles];
+#pragma unroll
+  for (int i = 0; i < kWaveTiles; ++i)
+#pragma unroll
+    for (int j = 0; j < kWaveTiles; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
+
+  // FP8 scales of the current stage (set 0) and of the next one (set 1, loaded one stage
+  // ahead and moved to set 0 at the stage boundary).
+  int sa[2][kWaveTiles], sb[2][kWaveTiles];
+  auto scales = [&](int ks, int set) {
+#pragma unroll
+    for (int i = 0; i < kWaveTiles; ++i) {
+      if constexpr (KIND == kFP8) {
+        sa[set][i] = sx[int64_t(ks) * Mp + m0 + wm * 128 + i * 32 + l32];
+        sb[set][i] = sy[int64_t(ks) * Np + n0 + wn * 128 + i * 32 + l32];
+      } else {
+        sa[set][i] = sb[set][i] = 0;
+      }
+    }
+  };
+  // DMAs of the next stage ride in the first 4 MFMA groups of a stage, 4 per group, so they
+  // have most of the stage to land.
+  auto dma_slot = [&](int group, int ks_next, int buf) {
+    if (group < 4) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dma(4 * group + e, ks_next, buf);
+    }
+  };
+
+#pragma unroll
+  for (int d = 0; d < 16; ++d) dma(d, 0, 0);
+  scales(0, 0);
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  __syncthreads();
+
+  if constexpr (KIND == kBF16X3) {
+    // Fragments: al, ah, bl single; bh double (it is used by the first and the last MFMA
+    // group of a step). Order per step: A = Al.Bh, B = Ah.Bl, C = Ah.Bh. Ah, Bl of step s
+    // are read at its start (used from group B on); Al, Bh of step s + 1 right after group A.
+    F al[kWaveTiles], ah[kWaveTiles], bl[kWaveTiles], bh[2][kWaveTiles];
+#pragma unroll
+    for (int i = 0; i < kWaveTiles; ++i) {
+      al[i] = frag(0, arow + i * 32 * kRowB, 0, 1);
+      bh[0][i] = frag(0, brow + i * 32 * kRowB, 0, 0);
+    }
+    auto stage = [&](int ks, auto more_tag) {
+      constexpr bool kMore = decltype(more_tag)::value;
+      const int buf = ks & 1;
+#pragma unroll
+      for (int s = 0; s < P::kSteps; ++s) {
+#pragma unroll
+        for (int i = 0; i < kWaveTiles; ++i) {
+          ah[i] = frag(buf, arow + i * 32 * kRowB, s, 0);
+          bl[i] = frag(buf, brow + i * 32 * kRowB, s, 1);
+        }
+        CSMI_SCHED_FENCE();
+#pragma unroll
+        for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+          for (int j = 0; j < kWaveTiles; ++j) mma<KIND>(al[g], bh[s & 1][j], acc[g][j], 0, 0);
+          if (kMore && s == 0) dma_slot(g, ks + 1, buf ^ 1);
+          CSMI_SCHED_FENCE();
+        }
+        if (s + 1 < P::kSteps) {
+#pragma unroll
+          for (int i = 0; i < kWaveTiles; ++i) {
+            al[i] = frag(buf, arow + i * 32 * kRowB, s + 1, 1);
+            bh[(s + 1) & 1][i] = frag(buf, brow + i * 32 * kRowB, s + 1, 0);
+          }
+        }
+        CSMI_SCHED_FENCE();
+#pragma unroll
+        for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+          for (int j = 0; j < kWaveTiles; ++j) mma<KIND>(ah[g], bl[j], acc[g][j], 0, 0);
+          CSMI_SCHED_FENCE();
+        }
+#pragma unroll
+        for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+          for (int j = 0; j < kWaveTiles; ++j) mma<KIND>(ah[g], bh[s & 1][j], acc[g][j], 0, 0);
+          CSMI_SCHED_FENCE();
+        }
+      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (kMore) {
+#pragma unroll
+        for (int i = 0; i < kWaveTiles; ++i) {
+          al[i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 1);
+          bh[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
+        }
+      }
+    };
+    run_stages(stages, stage);
+  } else {
+    // Fragments double-buffered across the K steps of a stage (kSteps is even, so step 0
+    // of every stage uses set 0).
+    F fa[2][kWaveTiles], fb[2][kWaveTiles];
+#pragma unroll
+    for (int i = 0; i < kWaveTiles; ++i) {
+      fa[0][i] = frag(0, arow + i * 32 * kRowB, 0, 0);
+      fb[0][i] = frag(0, brow + i * 32 * kRowB, 0, 0);
+    }
+    auto stage = [&](int ks, auto more_tag) {
+      constexpr bool kMore = decltype(more_tag)::value;
+      const int buf = ks & 1;
+      if (kMore) scales(ks + 1, 1);
+#pragma unroll
+      for (int s = 0; s < P::kSteps; ++s) {
+        if (s + 1 < P::kSteps) {
+#pragma unroll
+          for (int i = 0; i < kWaveTiles; ++i) {
+            fa[(s + 1) & 1][i] = frag(buf, arow + i * 32 * kRowB, s + 1, 0);
+            fb[(s + 1) & 1][i] = frag(buf, brow + i * 32 * kRowB, s + 1, 0);
+          }
+        }
+        CSMI_SCHED_FENCE();
+#pragma unroll
+        for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+          for (int j = 0; j < kWaveTiles; ++j)
+            mma<KIND>(fa[s & 1][g], fb[s & 1][j], acc[g][j], sa[0][g], sb[0][j]);
+          if (kMore) dma_slot(s * kWaveTiles + g, ks + 1, buf ^ 1);
+          CSMI_SCHED_FENCE();
+        }
+      }
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      __syncthreads();
+      if (kMore) {
+#pragma unroll
+        for (int i = 0; i < kWaveTiles; ++i) {
+          sa[0][i] = sa[1][i];
+          sb[0][i] = sb[1][i];
+          fa[0][i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 0);
+          fb[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
+        }
+      }
+    };
+    run_stages(stages, stage);
+  }
+
+  // acc[i][j][e]: row = 8*(e/4) + 4*half + e%4, col = l32 of the 32x32 tile (i, j)
+#pragma unroll
+  for (int i = 0; i < kWaveTiles; ++i)
+#pragma unroll
+    for (int j = 0; j < kWaveTiles; ++j)
+#pragma unroll
+      for (int e = 0; e < 16; ++e) {
+        const int r = m0 + wm * 128 + i * 32 + 8 * (e / 4) + 4 * half + (e % 4);
+        const int c = n0 + wn * 128 + j * 32 + l32;
+        if (r < M && c < N) C[int64_t(r) * ldc + c] = acc[i][j][e];
+      }
 }
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
+void run_f32(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
+             int ldc) {
+  const int64_t Mp = round_up(M, kTile), Np = round_up(N, kTile), Kp = round_up(K, kF32StageK);
+  const int64_t xbytes = Mp * Kp * 4, ybytes = Np * Kp * 4;
+  char *base = static_cast<char *>(stream_scratch(stream, static_cast<size_t>(xbytes + ybytes + 4096)));
+  float *x = reinterpret_cast<float *>(base), *y = reinterpret_cast<float *>(base + xbytes);
+  hipStream_t s = stream->stream;
+  hipLaunchKernelGGL(convert_f32_kernel<false>, dim3(unsigned((Kp + 31) / 32), unsigned(Mp / 32)), dim3(256), 0, s,
+                     A, M, K, lda, int(Mp), int(Kp), x);
+  hipLaunchKernelGGL(convert_f32_kernel<true>, dim3(unsigned((Kp + 31) / 32), unsigned(Np / 32)), dim3(256), 0, s,
+                     B, N, K, ldb, int(Np), int(Kp), y);
+  const int tiles_m = int(Mp / kTile), tiles_n = int(Np / kTile);
+  hipLaunchKernelGGL(gemm_f32_kernel, dim3(unsigned(tiles_m * tiles_n)), dim3(kThreads), 0, s, M, N, int(Kp),
+                     tiles_m, tiles_n, x, y, C, ldc);
+  CS_HIP(hipGetLastError());
+}
+
 template <int KIND>
-void run_gemm(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
-              int ldc) {
-  using T = Traits<KIND>;
-  const int64_t Mp = round_up(M, kTile), Np = round_up(N, kTile), Kp = round_up(K, T::kStageK);
-  const int64_t xbytes = Mp * Kp * T::kElem, ybytes = Np * Kp * T::kElem;
-  const int64_t parts = T::kParts;
-  int64_t total = parts * (xbytes + ybytes) + 4096;
+void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
+             int ldc) {
+  using P = Pipe<KIND>;
+  const int64_t Mp = round_up(M, kTile), Np = round_up(N, kTile), Kp = round_up(K, P::kK);
+  const int64_t rowb = Kp * P::kBytesPerK;
+  if (rowb * kTile >= (int64_t(1) << 31))
+    fail(CS_INVALID_VALUE, "K = " + std::to_string(K) + " is too large for this GEMM variant (K * bytes per "
+                           "value must stay below 8 MiB)");
+  const int64_t xbytes = Mp * rowb, ybytes = Np * rowb;
+  int64_t total = xbytes + ybytes + 4096;
   int64_t f32_stage = 0;
   if (KIND == kFP8) {
     f32_stage = (Mp > Np ? Mp : Np) * Kp * 4;  // FP32 staging for the padded, K-contiguous copy
     total += f32_stage + (Mp + Np) * (Kp / 128) + 4096;
   }
   char *base = static_cast<char *>(stream_scratch(stream, static_cast<size_t>(total)));
-  char *x0 = base, *y0 = x0 + xbytes;
-  char *x1 = parts == 2 ? y0 + ybytes : nullptr;
-  char *y1 = parts == 2 ? x1 + xbytes : nullptr;
+  char *x = base, *y = base + xbytes;
   uint8_t *sx = nullptr, *sy = nullptr;
   hipStream_t s = stream->stream;
   if (KIND == kFP8) {
     float *stage = reinterpret_cast<float *>(base + xbytes + ybytes);
     sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes + f32_stage);
     sy = sx + Mp * (Kp / 128);
-    dim3 ga((Kp + 31) / 32, (Mp + 31) / 32);
-    hipLaunchKernelGGL((convert_kernel<kF32, false>), ga, dim3(256), 0, s, A, M, K, lda, int(Mp), int(Kp), stage,
-                       nullptr);
+    hipLaunchKernelGGL(convert_f32_kernel<false>, dim3(unsigned(Kp / 32), unsigned(Mp / 32)), dim3(256), 0, s, A,
+                       M, K, lda, int(Mp), int(Kp), stage);
     hipLaunchKernelGGL(quantize_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) + 3) / 4)), dim3(256), 0, s, stage,
-                       int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x0), sx);
-    dim3 gb((Kp + 31) / 32, (Np + 31) / 32);
-    hipLaunchKernelGGL((convert_kernel<kF32, true>), gb, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), stage,
-                       nullptr);
+                       int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x), sx);
+    hipLaunchKernelGGL(convert_f32_kernel<true>, dim3(unsigned(Kp / 32), unsigned(Np / 32)), dim3(256), 0, s, B, N,
+                       K, ldb, int(Np), int(Kp), stage);
     hipLaunchKernelGGL(quantize_fp8_kernel, dim3(unsigned((Np * (Kp / 128) + 3) / 4)), dim3(256), 0, s, stage,
-                       int(Np), int(Kp), reinterpret_cast<uint8_t *>(y0), sy);
+                       int(Np), int(Kp), reinterpret_cast<uint8_t *>(y), sy);
   } else {
-    dim3 ga((Kp + 31) / 32, (Mp + 31) / 32);
-    hipLaunchKernelGGL((convert_kernel<KIND, false>), ga, dim3(256), 0, s, A, M, K, lda, int(Mp), int(Kp), x0, x1);
-    dim3 gb((Kp + 31) / 32, (Np + 31) / 32);
-    hipLaunchKernelGGL((convert_kernel<KIND, true>), gb, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), y0, y1);
+    constexpr bool kSplit = KIND == kBF16X3;
+    hipLaunchKernelGGL((pack_bf16_kernel<false, kSplit>), dim3(unsigned((Kp + 63) / 64), unsigned(Mp / 32)),
+                       dim3(256), 0, s, A, M, K, lda, int(Mp), int(Kp), x);
+    hipLaunchKernelGGL((pack_bf16_kernel<true, kSplit>), dim3(unsigned((Kp + 63) / 64), unsigned(Np / 32)),
+                       dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), y);
   }
-  const size_t lds = lds_bytes<KIND>();
-  static bool configured = false;
-  if (!configured) {
-    CS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gemm_nt_kernel<KIND>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    configured = true;
-  }
+  static const bool configured = [] {
+    CS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gemm_lds_kernel<KIND>),
+                               hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(kLdsBytes)));
+    return true;
+  }();
+  (void)configured;
   const unsigned grid = static_cast<unsigned>((Mp / kTile) * (Np / kTile));
-  hipLaunchKernelGGL(gemm_nt_kernel<KIND>, dim3(grid), dim3(kThreads), lds, s, M, N, int(Kp), x0, x1, y0, y1, sx,
-                     sy, C, ldc);
+  hipLaunchKernelGGL(gemm_lds_kernel<KIND>, dim3(grid), dim3(kThreads), kLdsBytes, s, M, N, int(Kp), int(Mp),
+                     int(Np), x, y, sx, sy, C, ldc);
   CS_HIP(hipGetLastError());
 }
 
@@ -392,10 +619,10 @@ extern "C" cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K
     set_device(stream->device);
     const int Ke = K == 0 ? 1 : K;  // K = 0: a zero-padded stage gives C = 0
     switch (variant) {
-      case CS_GEMM_FP32: run_gemm<kF32>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_BF16: run_gemm<kBF16>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_BF16X3: run_gemm<kBF16X3>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_FP8: run_gemm<kFP8>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_FP32: run_f32(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_BF16: run_lds<kBF16>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_BF16X3: run_lds<kBF16X3>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_FP8: run_lds<kFP8>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
     }
   });
 }

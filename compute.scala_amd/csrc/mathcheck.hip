@@ -25,7 +25,7 @@ __device__ __forceinline__ int64_t ordered(float f) {
 __device__ float eval32(int op, int mode, float x) {
   switch (op) {
     case CS_OP_EXP: return mode == CS_MATH_ACCURATE ? csmi_expf(x) : mode == CS_MATH_FAST ? __expf(x) : expf(x);
-    case CS_OP_TANH: return mode == CS_MATH_ACCURATE ? csmi_tanhf(x) : tanhf(x);
+    case CS_OP_TANH: return mode == CS_MATH_ACCURATE ? csmi_tanhf(x) : csmi_tanhf_p(x);
     case CS_OP_LOG: return mode == CS_MATH_FAST ? __logf(x) : logf(x);
     case CS_OP_SQRT: return sqrtf(x);
     default: return __builtin_nanf("");

@@ -1,9 +1,17 @@
-// Accurate FP32 exp and tanh for the precise math mode of generated kernels.
+// FP32 exp and tanh for the generated kernels' precise and accurate math modes.
 //
 // The reference's kernels call OpenCL's exp and tanh (OpenCLKernelBuilder.scala:295-325).
-// Here each result is within about 0.7 ulp of the exact value (checked over every FP32
-// input on the device), using only FMA-rate instructions:
 //
+// Precise mode (each result within 2 ulp, checked over every FP32 input on the device):
+//   csmi_tanhf_p(x), branch free: for |x| < 0.625 the odd polynomial x + x^3 P(x^2), P a
+//            degree-4 weighted least-squares fit of (tanh(x) - x) / x^3 (about 0.86 ulp in
+//            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} = 2^n 2^f from an
+//            extended-precision argument 2|x| log2(e) = n + f (log2 e as a float pair) and
+//            the hardware 2^f on |f| <= 1/2. |x| is clamped to 9.5 (tanh rounds to 1 beyond
+//            9.01), so no range checks are needed, and NaN is selected at the end.
+//   exp stays the device library's expf, which is already branch free.
+//
+// Accurate mode (each result within about 0.7 ulp, checked likewise), FMA-rate only:
 //   exp(x) = 2^n (1 + q),  n = rint(x log2 e),  r = x - n ln2 (Cody-Waite, ln2 in two
 //            parts, r kept as rh + rl), q = r + r^2 P(r) with a degree-5 least-squares
 //            polynomial P ~ (e^r - 1 - r) / r^2 on |r| <= ln2/2 carried as a double-float
@@ -13,6 +21,28 @@
 //            residual. No branch on |x|: small arguments keep full relative accuracy
 //            because E carries q directly.
 #pragma once
+
+__device__ __forceinline__ float csmi_tanhf_p(float x) {
+  const float a = fminf(fabsf(x), 9.5f);
+  // |x| >= 0.625: 1 - 2 / (e^{2a} + 1)
+  const float a2 = a + a;                                                 // exact
+  const float ph = a2 * 1.44269502f;                                      // log2(e), high part
+  const float pl = fmaf(a2, 1.92596299e-08f, fmaf(a2, 1.44269502f, -ph));  // + low part
+  const float n = rintf(ph);
+  const float f = (ph - n) + pl;
+  const float e = ldexpf(__builtin_amdgcn_exp2f(f), static_cast<int>(n));
+  const float big = fmaf(__builtin_amdgcn_rcpf(e + 1.0f), -2.0f, 1.0f);
+  // |x| < 0.625: a + a^3 P(a^2)
+  const float s = a * a;
+  float p = -0.005692214239388704f;
+  p = fmaf(p, s, 0.02062651701271534f);
+  p = fmaf(p, s, -0.0537353977560997f);
+  p = fmaf(p, s, 0.13331381976604462f);
+  p = fmaf(p, s, -0.3333328068256378f);
+  const float small = fmaf(a * s, p, a);
+  const float t = a < 0.625f ? small : big;
+  return x != x ? x : copysignf(t, x);
+}
 
 __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, float &ql) {
   n = rintf(x * 1.44269502f);

@@ -32,7 +32,7 @@ def test_c1_kernel_text(offline_tensors):
     src = T.kernel_source(r.closure, r.shape)
     assert src.count("const csmi_f4 a0_") == 3  # three wide loads
     assert "t4[u][v] = t1[u][v] * t3[u][v]" in src and "t7[u][v] = t4[u][v] + t6[u][v]" in src
-    body = src[src.index("void jit_kernel"):]
+    body = src[src.index("void csmi_fused_"):]
     assert "fma" not in body  # no contraction in the precise mode
     compiles(src)
 
@@ -144,3 +144,29 @@ def test_structural_key_ignores_leaf_identity(offline_tensors):
     assert key(a * b + a) == key(b * a + b)
     assert key(a * b + a) != key(a * b + b)
     assert key(a + b) != key(a.transpose() + b)
+
+
+def test_batch_source_offline(offline_tensors):
+    """The persistent batch's source (cs_batch_generate_source): one non-inlined device
+    function per expression over global-address-space pointers, a constant workgroup size,
+    and one switch case per expression; the text compiles for gfx950."""
+    from compute import batch as bm
+    from compute import workloads
+    T = offline_tensors
+    exprs = workloads.uneven_dags(T, n=6, seed=11, min_elements=100, max_elements=5000)
+    cfgs = [{"vector_width": 2, "items_per_thread": 2, "block_size": 512}] * len(exprs)
+    descs, keep = bm._descs(exprs, cfgs, capi.MATH_PRECISE)
+    src, items = ctypes.c_void_p(), ctypes.c_int64()
+    capi.check(capi.lib.cs_batch_generate_source(descs, len(exprs), 512, ctypes.byref(src), ctypes.byref(items)))
+    text = capi.take_string(src.value)
+    assert text.count("__device__ __noinline__ void expr_") == len(exprs)
+    assert "#define CSMI_G __attribute__((address_space(1)))" in text
+    assert "(tile) * (512u * 2u)" in text and "blockDim" not in text.split("batch_kernel")[0]
+    for i, e in enumerate(exprs):
+        assert "case %d: expr_%d(t" % (i, i) in text
+    # every tile of every expression is in exactly one work item
+    tiles = sum(-(-(e.out_elements // bm.effective_vector_width(e.shape, 2)) // (512 * 2)) for e in exprs)
+    assert 1 <= items.value <= tiles
+    compiles(text)
+    with pytest.raises(capi.InvalidWorkGroupSize):
+        capi.check(capi.lib.cs_batch_generate_source(descs, len(exprs), 96, ctypes.byref(src), None))
