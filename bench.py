@@ -486,6 +486,15 @@ def main():
         cpu = cpu_baseline_c2()
         log("cpu baseline: %s" % cpu)
     gbs = c2["gbs"]
+    # HBM bytes per launch of the headline kernel, from the FETCH_SIZE / WRITE_SIZE passes
+    # (scaled by what the counters read for a copy of known size) recorded under profiles/.
+    traffic, traffic_source = None, None
+    tp = os.path.join(ROOT, "profiles", "c2_hbm_traffic.json")
+    if os.path.exists(tp):
+        with open(tp) as f:
+            t = json.load(f)
+        traffic = t["traffic_bytes"]
+        traffic_source = "profiles/c2_hbm_traffic.json, launch configuration %s" % (t["config"],)
     line = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -505,7 +514,8 @@ def main():
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "launch_config": c2["tuning"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None},
+                     "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": traffic_source},
         "cpu_baseline": cpu,
         "also": also,
     }
