@@ -182,7 +182,12 @@ def bench_c1(T, capi, steps, sets=16):
     from compute import tensors as tz
     ta, tb, tc = (tz.CachedTensor(T, (1024, 1024), 0.0, a[0][j].retain()) for j in range(3))
     e = ta * tb + tc
-    f, leaves, _ = T.compile(e.closure, e.shape)
+    # launch configuration from the same autotuner sweep as the headline kernel
+    from compute import autotune
+    cands = autotune.candidates(1024, vector_widths=(4, 8), items=(1, 2, 4), blocks=(256, 512), nontemporal=(0, 1))
+    results = autotune.tune_fused(T, e.closure, e.shape, [[a[i][j].ptr for j in range(3)] for i in range(sets)],
+                                  [o.ptr for o in outs], cands, reps=8)
+    f, leaves = autotune.compile_with(T, e.closure, e.shape, results[0][0])
     ptrs = [(ctypes.c_void_p * 3)(*[a[i][j].ptr for j in range(3)]) for i in range(sets)]
     for i in range(sets):
         capi.check(lib.cs_fused_launch(T.stream, f, ptrs[i], 3, outs[i].ptr))
@@ -199,7 +204,7 @@ def bench_c1(T, capi, steps, sets=16):
     timer.measure(lambda: lib.cs_graph_launch(T.stream, g2), 2)
     ms_sum = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g2)), reps)
     L = reps * sets
-    res = {"fma_gbs": 16 * n * L / (ms_fma * 1e-3) / 1e9, "fma_us": ms_fma * 1e3 / L,
+    res = {"fma_gbs": 16 * n * L / (ms_fma * 1e-3) / 1e9, "fma_us": ms_fma * 1e3 / L, "fma_config": results[0][0],
            "sum_gbs": (4 * n + 4) * L / (ms_sum * 1e-3) / 1e9, "sum_us": ms_sum * 1e3 / L}
     for g in (g1, g2):
         lib.cs_graph_release(g)
@@ -438,6 +443,7 @@ def main():
     ap.add_argument("--out-dir", default=None)
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
+    ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")
@@ -475,10 +481,11 @@ def main():
         also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16))
         log("c3: %s" % also["c3_rows_sum_max_total"])
     if "c4" in only:
-        also["c4_gemm_4096"] = bench_gemm(T, capi, [("fp32", capi.GEMM_FP32), ("bf16", capi.GEMM_BF16),
-                                                     ("bf16x3", capi.GEMM_BF16X3), ("fp8", capi.GEMM_FP8)])
+        kinds = {"fp32": capi.GEMM_FP32, "bf16": capi.GEMM_BF16, "bf16x3": capi.GEMM_BF16X3, "fp8": capi.GEMM_FP8}
+        also["c4_gemm_4096"] = bench_gemm(T, capi, [(k, kinds[k]) for k in args.gemm.split(",")])
         g = also["c4_gemm_4096"]
-        g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
+        if "bf16x3" in g and "fp32" in g:
+            g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
     if "c5" in only:
         also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir)
     cpu = None

@@ -72,113 +72,219 @@ __device__ __forceinline__ uint16_t bf16_rne(float x) {
 }
 __device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-// FP32 copy of a row-major matrix (rows x cols, leading dimension ld) into a zero-padded
-// [rows_p][cols_p] operand; kTranspose reads the source as its transpose (B -> Y).
-template <bool kTranspose>
-__global__ __launch_bounds__(256) void convert_f32_kernel(const float *__restrict__ src, int rows, int cols, int ld,
-                                                          int rows_p, int cols_p, float *__restrict__ out) {
-  __shared__ float tile[32][33];
-  const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
-  const int tx = threadIdx.x % 32, ty = threadIdx.x / 32;  // 32 x 8
-  for (int i = ty; i < 32; i += 8) {
-    float v = 0.0f;
-    if (kTranspose) {  // tile[i][tx] = source (c0 + i, r0 + tx), stored at output (r0 + tx, c0 + i)
-      const int sr = c0 + i, sc = r0 + tx;
-      if (sr < cols && sc < rows) v = src[int64_t(sr) * ld + sc];
-    } else {
-      const int r = r0 + i, c = c0 + tx;
-      if (r < rows && c < cols) v = src[int64_t(r) * ld + c];
+// FP32 operands for the FP32 kernel, zero padded to [rows_p][kp]. A (K contiguous): one
+// thread per 4 consecutive K values of a row, 16-byte loads when the source is 16-byte
+// aligned (`vec`) and 16-byte stores.
+__global__ __launch_bounds__(256) void pack_rows_f32_kernel(const float *__restrict__ src, int rows, int cols, int ld,
+                                                            int rows_p, int kp, int vec, float *__restrict__ out) {
+  const int chunks = kp / 4;
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(rows_p) * chunks) return;
+  const int r = static_cast<int>(idx / chunks), k = static_cast<int>(idx % chunks) * 4;
+  const float *p = src + int64_t(r) * ld + k;
+  float4 v = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  if (vec && r < rows && k + 4 <= cols) {
+    v = *reinterpret_cast<const float4 *>(p);
+  } else if (r < rows) {
+    v.x = k < cols ? p[0] : 0.0f;
+    v.y = k + 1 < cols ? p[1] : 0.0f;
+    v.z = k + 2 < cols ? p[2] : 0.0f;
+    v.w = k + 3 < cols ? p[3] : 0.0f;
+  }
+  *reinterpret_cast<float4 *>(out + int64_t(r) * kp + k) = v;
+}
+
+// B (K x N, N contiguous) -> Y[rows_p][kp], row n = column n of B: 64 K x 64 N tiles through
+// LDS, 16-byte loads along N (`vec`) and 16-byte stores along K.
+__global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restrict__ src, int n_src, int k_src,
+                                                            int ld, int rows_p, int kp, int vec,
+                                                            float *__restrict__ out) {
+  __shared__ float tile[64][65];  // [k][n]
+  const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64, t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = t / 16 + 16 * i, n = (t % 16) * 4;
+    const float *p = src + int64_t(k0 + k) * ld + (n0 + n);
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k0 + k < k_src) {
+      if (vec && n0 + n + 4 <= n_src) {
+        a = *reinterpret_cast<const float4 *>(p);
+      } else {
+        a.x = n0 + n < n_src ? p[0] : 0.0f;
+        a.y = n0 + n + 1 < n_src ? p[1] : 0.0f;
+        a.z = n0 + n + 2 < n_src ? p[2] : 0.0f;
+        a.w = n0 + n + 3 < n_src ? p[3] : 0.0f;
+      }
     }
-    tile[i][tx] = v;
+    tile[k][n] = a.x;
+    tile[k][n + 1] = a.y;
+    tile[k][n + 2] = a.z;
+    tile[k][n + 3] = a.w;
   }
   __syncthreads();
-  for (int i = ty; i < 32; i += 8) {
-    const int r = r0 + i, c = c0 + tx;
-    if (r >= rows_p || c >= cols_p) continue;
-    out[int64_t(r) * cols_p + c] = kTranspose ? tile[tx][i] : tile[i][tx];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int n = t / 16 + 16 * i, c = t % 16, k = k0 + 4 * c;
+    if (n0 + n >= rows_p || k >= kp) continue;
+    const float4 v = make_float4(tile[4 * c][n], tile[4 * c + 1][n], tile[4 * c + 2][n], tile[4 * c + 3][n]);
+    *reinterpret_cast<float4 *>(out + int64_t(n0 + n) * kp + k) = v;
   }
 }
 
-// BF16 operand rows for the LDS pipeline: output row r holds source row r (kTranspose: source
-// column r) along K. kSplit == false: Kp BF16 values per row. kSplit == true: per 32-wide K
+// Operand rows for the LDS pipeline. BF16: Kp values per row. Split (BF16X3): per 32-wide K
 // block, 32 BF16 high parts then 32 BF16 low parts (128 bytes), so that one K stage of a row
-// is 128 contiguous bytes in every variant. A block covers 32 rows x 64 K values; each
-// thread writes 8 consecutive K values.
-template <bool kTranspose, bool kSplit>
-__global__ __launch_bounds__(256) void pack_bf16_kernel(const float *__restrict__ src, int rows, int cols, int ld,
-                                                        int rows_p, int kp, char *__restrict__ out) {
-  __shared__ float tile[64][33];  // [k][row]
-  const int r0 = blockIdx.y * 32, k0 = blockIdx.x * 64;
-  const int t = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    float v = 0.0f;
-    if (kTranspose) {  // source row k (contiguous along r)
-      const int r = t % 32, k = t / 32 + 8 * i;
-      if (k0 + k < cols && r0 + r < rows) v = src[int64_t(k0 + k) * ld + (r0 + r)];
-      tile[k][r] = v;
-    } else {  // source row r (contiguous along k)
-      const int k = t % 64, r = t / 64 + 4 * i;
-      if (r0 + r < rows && k0 + k < cols) v = src[int64_t(r0 + r) * ld + (k0 + k)];
-      tile[k][r] = v;
-    }
-  }
-  __syncthreads();
-  const int r = t / 8, c = t % 8;
-  const int k = k0 + 8 * c;
-  if (r0 + r >= rows_p || k >= kp) return;
+// is 128 contiguous bytes in every variant.
+__device__ __forceinline__ void store_bf16x8(const float (&v)[8], char *row, int k, bool split) {
   uint16_t hi[8], lo[8];
 #pragma unroll
   for (int e = 0; e < 8; ++e) {
-    const float v = tile[8 * c + e][r];
-    hi[e] = bf16_rne(v);
-    lo[e] = bf16_rne(v - bf16_to_f32(hi[e]));
+    hi[e] = bf16_rne(v[e]);
+    lo[e] = bf16_rne(v[e] - bf16_to_f32(hi[e]));
   }
   auto pack = [](const uint16_t *h) {
     return i32x4{int(h[0] | (uint32_t(h[1]) << 16)), int(h[2] | (uint32_t(h[3]) << 16)),
                  int(h[4] | (uint32_t(h[5]) << 16)), int(h[6] | (uint32_t(h[7]) << 16))};
   };
-  if (kSplit) {
-    char *row = out + int64_t(r0 + r) * (int64_t(kp) * 4);
+  if (split) {
     char *p = row + (k / 32) * 128 + (k % 32) * 2;
     *reinterpret_cast<i32x4 *>(p) = pack(hi);
     *reinterpret_cast<i32x4 *>(p + 64) = pack(lo);
   } else {
-    char *row = out + int64_t(r0 + r) * (int64_t(kp) * 2);
     *reinterpret_cast<i32x4 *>(row + int64_t(k) * 2) = pack(hi);
   }
 }
 
-// FP8 quantisation of a [rows][cols] FP32 matrix (K-contiguous, zero padded) in blocks of
-// 128 along each row: one wave per block, scale 2^e with 2^e >= amax / 448. Scales are
-// written K-block-major, scales[kb * rows + r], so that a wave reads 32 consecutive bytes.
-__global__ __launch_bounds__(256) void quantize_fp8_kernel(const float *__restrict__ src, int rows, int cols,
-                                                           uint8_t *__restrict__ q, uint8_t *__restrict__ scales) {
-  const int wave = (blockIdx.x * 256 + threadIdx.x) / 64;
-  const int lane = threadIdx.x % 64;
-  const int blocks_per_row = cols / 128;
-  if (wave >= rows * blocks_per_row) return;
-  const int r = wave / blocks_per_row, kb = wave % blocks_per_row;
-  const float *p = src + int64_t(r) * cols + kb * 128;
-  float x0 = p[lane], x1 = p[lane + 64];
+// A (rows x cols, K contiguous) -> X: one thread per 8 consecutive K values of a row, two
+// 16-byte loads when the source is 16-byte aligned (`vec`), and one (split: two) 16-byte
+// stores. Rows and K values beyond the source are zero.
+template <bool kSplit>
+__global__ __launch_bounds__(256) void pack_rows_bf16_kernel(const float *__restrict__ src, int rows, int cols,
+                                                             int ld, int rows_p, int kp, int vec,
+                                                             char *__restrict__ out) {
+  const int chunks = kp / 8;
+  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (idx >= int64_t(rows_p) * chunks) return;
+  const int r = static_cast<int>(idx / chunks), k = static_cast<int>(idx % chunks) * 8;
+  float v[8];
+  const float *p = src + int64_t(r) * ld + k;
+  if (vec && r < rows && k + 8 <= cols) {
+    const float4 a = *reinterpret_cast<const float4 *>(p), b = *reinterpret_cast<const float4 *>(p + 4);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w; v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (r < rows && k + e < cols) ? p[e] : 0.0f;
+  }
+  store_bf16x8(v, out + int64_t(r) * (int64_t(kp) * (kSplit ? 4 : 2)), k, kSplit);
+}
+
+// B (K x N, N contiguous) -> Y (row n holds column n of B along K): 64 K x 64 N tiles through
+// LDS; loads are 16 bytes along N (`vec`), stores 16 bytes along K.
+template <bool kSplit>
+__global__ __launch_bounds__(256) void pack_cols_bf16_kernel(const float *__restrict__ src, int n_src, int k_src,
+                                                             int ld, int rows_p, int kp, int vec,
+                                                             char *__restrict__ out) {
+  __shared__ float tile[64][65];  // [k][n]
+  const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64, t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int k = t / 16 + 16 * i, n = (t % 16) * 4;
+    const float *p = src + int64_t(k0 + k) * ld + (n0 + n);
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k0 + k < k_src) {
+      if (vec && n0 + n + 4 <= n_src) {
+        a = *reinterpret_cast<const float4 *>(p);
+      } else {
+        a.x = n0 + n < n_src ? p[0] : 0.0f;
+        a.y = n0 + n + 1 < n_src ? p[1] : 0.0f;
+        a.z = n0 + n + 2 < n_src ? p[2] : 0.0f;
+        a.w = n0 + n + 3 < n_src ? p[3] : 0.0f;
+      }
+    }
+    tile[k][n] = a.x;
+    tile[k][n + 1] = a.y;
+    tile[k][n + 2] = a.z;
+    tile[k][n + 3] = a.w;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    const int n = t / 8 + 32 * i, c = t % 8, k = k0 + 8 * c;
+    if (n0 + n >= rows_p || k >= kp) continue;
+    float v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = tile[8 * c + e][n];
+    store_bf16x8(v, out + int64_t(n0 + n) * (int64_t(kp) * (kSplit ? 4 : 2)), k, kSplit);
+  }
+}
+
+// FP8 quantisation of 128 values: the scale is 2^e with e the smallest integer such that
+// 448 * 2^e >= amax (|x| / 2^e <= 448, the e4m3 maximum); amax = 0 gives e = 0. Returns the
+// E8M0 scale byte e + 127.
+__device__ __forceinline__ int fp8_scale_exponent(float amax) {
+  int e = 0;
+  if (amax > 0.0f) {
+    int ex;
+    frexpf(amax / 448.0f, &ex);  // amax/448 = f * 2^ex, f in [0.5, 1)
+    e = ex;
+    if (ldexpf(1.0f, e - 1) >= amax / 448.0f) e -= 1;
+  }
+  return e < -127 ? -127 : (e > 127 ? 127 : e);
+}
+__device__ __forceinline__ uint8_t fp8_e4m3(float x) {
+  return static_cast<uint8_t>(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+}
+
+// A (rows x cols, K contiguous) -> X[rows_p][kp] e4m3 plus scales[kb * rows_p + r]: one wave
+// per 128-wide block of a row, two consecutive values per lane.
+__global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__restrict__ src, int rows, int cols,
+                                                                int ld, int rows_p, int kp,
+                                                                uint8_t *__restrict__ q,
+                                                                uint8_t *__restrict__ scales) {
+  const int wave = (blockIdx.x * 256 + threadIdx.x) / 64, lane = threadIdx.x % 64;
+  const int blocks = kp / 128;
+  if (wave >= rows_p * blocks) return;
+  const int r = wave / blocks, kb = wave % blocks, k = kb * 128 + 2 * lane;
+  const float *p = src + int64_t(r) * ld + k;
+  const float x0 = (r < rows && k < cols) ? p[0] : 0.0f;
+  const float x1 = (r < rows && k + 1 < cols) ? p[1] : 0.0f;
   float m = fmaxf(fabsf(x0), fabsf(x1));
   for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-  // e = smallest integer with 448 * 2^e >= m (so |x| / 2^e <= 448); m = 0 -> e = 0
-  int e = 0;
-  if (m > 0.0f) {
-    int ex;
-    frexpf(m / 448.0f, &ex);  // m/448 = f * 2^ex, f in [0.5, 1)
-    e = ex;
-    if (ldexpf(1.0f, e - 1) >= m / 448.0f) e -= 1;
-  }
-  e = e < -127 ? -127 : (e > 127 ? 127 : e);
+  const int e = fp8_scale_exponent(m);
   const float inv = ldexpf(1.0f, -e);
-  uint8_t *qp = q + int64_t(r) * cols + kb * 128;
-  int packed0 = __builtin_amdgcn_cvt_pk_fp8_f32(x0 * inv, 0.0f, 0, false);
-  int packed1 = __builtin_amdgcn_cvt_pk_fp8_f32(x1 * inv, 0.0f, 0, false);
-  qp[lane] = static_cast<uint8_t>(packed0 & 0xff);
-  qp[lane + 64] = static_cast<uint8_t>(packed1 & 0xff);
-  if (lane == 0) scales[int64_t(kb) * rows + r] = static_cast<uint8_t>(e + 127);
+  const uint16_t two = uint16_t(fp8_e4m3(x0 * inv) | (uint16_t(fp8_e4m3(x1 * inv)) << 8));
+  *reinterpret_cast<uint16_t *>(q + int64_t(r) * kp + k) = two;
+  if (lane == 0) scales[int64_t(kb) * rows_p + r] = static_cast<uint8_t>(e + 127);
+}
+
+// B (K x N, N contiguous) -> Y[rows_p][kp] e4m3 (row n = column n of B) plus
+// scales[kb * rows_p + n]: a block transposes 128 K x 64 N through LDS, then each wave
+// quantises 16 columns, one 128-wide block each.
+__global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__restrict__ src, int n_src,
+                                                                int k_src, int ld, int rows_p, int kp,
+                                                                uint8_t *__restrict__ q,
+                                                                uint8_t *__restrict__ scales) {
+  __shared__ float tile[128][65];  // [k][n]
+  const int n0 = blockIdx.y * 64, kb = blockIdx.x, k0 = kb * 128, t = threadIdx.x;
+#pragma unroll
+  for (int i = 0; i < 32; ++i) {
+    const int k = t / 64 + 4 * i, n = t % 64;
+    tile[k][n] = (k0 + k < k_src && n0 + n < n_src) ? src[int64_t(k0 + k) * ld + (n0 + n)] : 0.0f;
+  }
+  __syncthreads();
+  const int wave = t / 64, lane = t % 64;
+  for (int j = 0; j < 16; ++j) {
+    const int n = wave * 16 + j;
+    if (n0 + n >= rows_p) break;
+    const float x0 = tile[lane][n], x1 = tile[lane + 64][n];
+    float m = fmaxf(fabsf(x0), fabsf(x1));
+    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const int e = fp8_scale_exponent(m);
+    const float inv = ldexpf(1.0f, -e);
+    uint8_t *row = q + int64_t(n0 + n) * kp + k0;
+    row[lane] = fp8_e4m3(x0 * inv);
+    row[lane + 64] = fp8_e4m3(x1 * inv);
+    if (lane == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
+  }
 }
 
 // ---------------------------------------------------------------------------- FP32 kernel
@@ -443,6 +549,19 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
             al[i] = frag(buf, arow + i * 32 * kRowB, s + 1, 1);
             bh[(s + 1) & 1][i] = frag(buf, brow + i * 32 * kRowB, s + 1, 0);
           }
+        } else {
+          // Every read of this stage's buffer has been issued (Ah, Bl above): publish the next
+          // stage (own DMAs landed, then the barrier) and read its first fragments while the
+          // remaining two groups run.
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          __syncthreads();
+          if (kMore) {
+#pragma unroll
+            for (int i = 0; i < kWaveTiles; ++i) {
+              al[i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 1);
+              bh[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
+            }
+          }
         }
         CSMI_SCHED_FENCE();
 #pragma unroll
@@ -456,15 +575,6 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
 #pragma unroll
           for (int j = 0; j < kWaveTiles; ++j) mma<KIND>(ah[g], bh[s & 1][j], acc[g][j], 0, 0);
           CSMI_SCHED_FENCE();
-        }
-      }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();
-      if (kMore) {
-#pragma unroll
-        for (int i = 0; i < kWaveTiles; ++i) {
-          al[i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 1);
-          bh[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
         }
       }
     };
@@ -490,6 +600,19 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
             fa[(s + 1) & 1][i] = frag(buf, arow + i * 32 * kRowB, s + 1, 0);
             fb[(s + 1) & 1][i] = frag(buf, brow + i * 32 * kRowB, s + 1, 0);
           }
+        } else {
+          // The last step's fragments were read during the previous step, so this stage's
+          // buffer is no longer read: publish the next stage (own DMAs and scale loads landed,
+          // then the barrier) and read its first fragments while this step's MFMAs run.
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+          __syncthreads();
+          if (kMore) {
+#pragma unroll
+            for (int i = 0; i < kWaveTiles; ++i) {
+              fa[0][i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 0);
+              fb[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
+            }
+          }
         }
         CSMI_SCHED_FENCE();
 #pragma unroll
@@ -501,15 +624,11 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
           CSMI_SCHED_FENCE();
         }
       }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-      __syncthreads();
       if (kMore) {
 #pragma unroll
         for (int i = 0; i < kWaveTiles; ++i) {
           sa[0][i] = sa[1][i];
           sb[0][i] = sb[1][i];
-          fa[0][i] = frag(buf ^ 1, arow + i * 32 * kRowB, 0, 0);
-          fb[0][i] = frag(buf ^ 1, brow + i * 32 * kRowB, 0, 0);
         }
       }
     };
@@ -538,10 +657,12 @@ void run_f32(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
   char *base = static_cast<char *>(stream_scratch(stream, static_cast<size_t>(xbytes + ybytes + 4096)));
   float *x = reinterpret_cast<float *>(base), *y = reinterpret_cast<float *>(base + xbytes);
   hipStream_t s = stream->stream;
-  hipLaunchKernelGGL(convert_f32_kernel<false>, dim3(unsigned((Kp + 31) / 32), unsigned(Mp / 32)), dim3(256), 0, s,
-                     A, M, K, lda, int(Mp), int(Kp), x);
-  hipLaunchKernelGGL(convert_f32_kernel<true>, dim3(unsigned((Kp + 31) / 32), unsigned(Np / 32)), dim3(256), 0, s,
-                     B, N, K, ldb, int(Np), int(Kp), y);
+  const int vec_a = (lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0) ? 1 : 0;
+  const int vec_b = (ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) ? 1 : 0;
+  hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
+                     lda, int(Mp), int(Kp), vec_a, x);
+  hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256), 0, s, B,
+                     N, K, ldb, int(Np), int(Kp), vec_b, y);
   const int tiles_m = int(Mp / kTile), tiles_n = int(Np / kTile);
   hipLaunchKernelGGL(gemm_f32_kernel, dim3(unsigned(tiles_m * tiles_n)), dim3(kThreads), 0, s, M, N, int(Kp),
                      tiles_m, tiles_n, x, y, C, ldc);
@@ -559,33 +680,26 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
                            "value must stay below 8 MiB)");
   const int64_t xbytes = Mp * rowb, ybytes = Np * rowb;
   int64_t total = xbytes + ybytes + 4096;
-  int64_t f32_stage = 0;
-  if (KIND == kFP8) {
-    f32_stage = (Mp > Np ? Mp : Np) * Kp * 4;  // FP32 staging for the padded, K-contiguous copy
-    total += f32_stage + (Mp + Np) * (Kp / 128) + 4096;
-  }
+  if (KIND == kFP8) total += (Mp + Np) * (Kp / 128) + 4096;
   char *base = static_cast<char *>(stream_scratch(stream, static_cast<size_t>(total)));
   char *x = base, *y = base + xbytes;
   uint8_t *sx = nullptr, *sy = nullptr;
   hipStream_t s = stream->stream;
+  const int vec_a = (lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0) ? 1 : 0;
+  const int vec_b = (ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) ? 1 : 0;
   if (KIND == kFP8) {
-    float *stage = reinterpret_cast<float *>(base + xbytes + ybytes);
-    sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes + f32_stage);
+    sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes);
     sy = sx + Mp * (Kp / 128);
-    hipLaunchKernelGGL(convert_f32_kernel<false>, dim3(unsigned(Kp / 32), unsigned(Mp / 32)), dim3(256), 0, s, A,
-                       M, K, lda, int(Mp), int(Kp), stage);
-    hipLaunchKernelGGL(quantize_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) + 3) / 4)), dim3(256), 0, s, stage,
-                       int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x), sx);
-    hipLaunchKernelGGL(convert_f32_kernel<true>, dim3(unsigned(Kp / 32), unsigned(Np / 32)), dim3(256), 0, s, B, N,
-                       K, ldb, int(Np), int(Kp), stage);
-    hipLaunchKernelGGL(quantize_fp8_kernel, dim3(unsigned((Np * (Kp / 128) + 3) / 4)), dim3(256), 0, s, stage,
-                       int(Np), int(Kp), reinterpret_cast<uint8_t *>(y), sy);
+    hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) + 3) / 4)), dim3(256), 0, s, A, M,
+                       K, lda, int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x), sx);
+    hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Kp / 128), unsigned(Np / 64)), dim3(256), 0, s, B, N,
+                       K, ldb, int(Np), int(Kp), reinterpret_cast<uint8_t *>(y), sy);
   } else {
     constexpr bool kSplit = KIND == kBF16X3;
-    hipLaunchKernelGGL((pack_bf16_kernel<false, kSplit>), dim3(unsigned((Kp + 63) / 64), unsigned(Mp / 32)),
-                       dim3(256), 0, s, A, M, K, lda, int(Mp), int(Kp), x);
-    hipLaunchKernelGGL((pack_bf16_kernel<true, kSplit>), dim3(unsigned((Kp + 63) / 64), unsigned(Np / 32)),
-                       dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), y);
+    hipLaunchKernelGGL(pack_rows_bf16_kernel<kSplit>, dim3(unsigned((Mp * (Kp / 8) + 255) / 256)), dim3(256), 0, s,
+                       A, M, K, lda, int(Mp), int(Kp), vec_a, x);
+    hipLaunchKernelGGL(pack_cols_bf16_kernel<kSplit>, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256),
+                       0, s, B, N, K, ldb, int(Np), int(Kp), vec_b, y);
   }
   static const bool configured = [] {
     CS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gemm_lds_kernel<KIND>),

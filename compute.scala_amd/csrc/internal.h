@@ -128,7 +128,7 @@ struct cs_device {
 struct cs_stream {
   cs_device *device = nullptr;
   hipStream_t stream = nullptr;
-  // Scratch owned by this stream (reduction tickets and partials, GEMM operand copies).
+  // Scratch owned by this stream (reduction partials, GEMM operand copies).
   // Work on one stream is ordered, so calls on it may share the scratch.
   void *scratch = nullptr;
   size_t scratch_bytes = 0;

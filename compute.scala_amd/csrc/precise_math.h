@@ -5,10 +5,11 @@
 // Precise mode (each result within 2 ulp, checked over every FP32 input on the device):
 //   csmi_tanhf_p(x), branch free: for |x| < 0.625 the odd polynomial x + x^3 P(x^2), P a
 //            degree-4 weighted least-squares fit of (tanh(x) - x) / x^3 (about 0.86 ulp in
-//            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} = 2^n 2^f from an
-//            extended-precision argument 2|x| log2(e) = n + f (log2 e as a float pair) and
-//            the hardware 2^f on |f| <= 1/2. |x| is clamped to 9.5 (tanh rounds to 1 beyond
-//            9.01), so no range checks are needed, and NaN is selected at the end.
+//            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} = 2^(th + tl) from the
+//            extended-precision argument th + tl = 2|x| log2(e) (log2 e as a float pair):
+//            E = 2^th (1 + tl ln 2) with the hardware 2^th (th in [1.8, 27.4], normal results).
+//            |x| is clamped to 9.5 (tanh rounds to 1 beyond 9.01) by a compare that keeps NaN,
+//            so no range checks are needed.
 //   exp stays the device library's expf, which is already branch free.
 //
 // Accurate mode (each result within about 0.7 ulp, checked likewise), FMA-rate only:
@@ -23,14 +24,13 @@
 #pragma once
 
 __device__ __forceinline__ float csmi_tanhf_p(float x) {
-  const float a = fminf(fabsf(x), 9.5f);
-  // |x| >= 0.625: 1 - 2 / (e^{2a} + 1)
-  const float a2 = a + a;                                                 // exact
-  const float ph = a2 * 1.44269502f;                                      // log2(e), high part
-  const float pl = fmaf(a2, 1.92596299e-08f, fmaf(a2, 1.44269502f, -ph));  // + low part
-  const float n = rintf(ph);
-  const float f = (ph - n) + pl;
-  const float e = ldexpf(__builtin_amdgcn_exp2f(f), static_cast<int>(n));
+  const float ax = fabsf(x);
+  const float a = !(ax > 9.5f) ? ax : 9.5f;  // keeps NaN; tanh rounds to 1 beyond 9.01
+  // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) = th + tl in [1.8, 27.4]
+  const float th = a * 2.88539004f;                                        // 2 log2(e), high part
+  const float tl = fmaf(a, 3.85192598e-08f, fmaf(a, 2.88539004f, -th));  // + low part
+  const float e0 = __builtin_amdgcn_exp2f(th);
+  const float e = fmaf(e0, tl * 0.693147182f, e0);  // 2^(th + tl), first order in tl (|tl| < 2^-19)
   const float big = fmaf(__builtin_amdgcn_rcpf(e + 1.0f), -2.0f, 1.0f);
   // |x| < 0.625: a + a^3 P(a^2)
   const float s = a * a;
@@ -40,8 +40,7 @@ __device__ __forceinline__ float csmi_tanhf_p(float x) {
   p = fmaf(p, s, 0.13331381976604462f);
   p = fmaf(p, s, -0.3333328068256378f);
   const float small = fmaf(a * s, p, a);
-  const float t = a < 0.625f ? small : big;
-  return x != x ? x : copysignf(t, x);
+  return copysignf(a < 0.625f ? small : big, x);  // NaN: a is NaN, so is big
 }
 
 __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, float &ql) {

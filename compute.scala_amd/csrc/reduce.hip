@@ -92,37 +92,58 @@ __device__ __forceinline__ void block_reduce(Acc &a, float &m) {
   }
 }
 
-// One workgroup of kFinish threads adds partials[0..n) in FP64 in a fixed order.
+// One workgroup of kFinish threads adds partials[0..n) in FP64 in a fixed order: thread t
+// adds pairs 2t, 2t + 2 kFinish, ... (16-byte loads, four in flight), then a fixed xor-shuffle
+// tree inside each wave and the 16 waves in order.
 constexpr int kFinish = 1024;
+typedef double d2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(kFinish) void finish_sum_kernel(const double *__restrict__ partials, int64_t n,
                                                              float *out, double *out64) {
-  __shared__ double l64[kFinish];
+  __shared__ double lw[kFinish / 64];
   double acc = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kFinish) acc += partials[i];
-  l64[threadIdx.x] = acc;
-  __syncthreads();
-  for (int off = kFinish / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) l64[threadIdx.x] += l64[threadIdx.x + off];
-    __syncthreads();
+  const int64_t pairs = n / 2;
+  const d2 *p2 = reinterpret_cast<const d2 *>(partials);  // partials are 16-byte aligned
+  int64_t i = threadIdx.x;
+  for (; i + 3 * kFinish < pairs; i += 4 * kFinish) {
+    d2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) v[k] = p2[i + k * kFinish];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) acc += v[k].x + v[k].y;
   }
+  for (; i < pairs; i += kFinish) {
+    const d2 v = p2[i];
+    acc += v.x + v.y;
+  }
+  if (threadIdx.x == 0 && (n & 1)) acc += partials[n - 1];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = acc;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (out) *out = float(l64[0]);
-    if (out64) *out64 = l64[0];
+    double t = lw[0];
+#pragma unroll
+    for (int w = 1; w < kFinish / 64; ++w) t += lw[w];
+    if (out) *out = float(t);
+    if (out64) *out64 = t;
   }
 }
 
 __global__ __launch_bounds__(kFinish) void finish_max_kernel(const float *__restrict__ partials, int64_t n,
                                                              float *out) {
-  __shared__ float lm[kFinish];
+  __shared__ float lw[kFinish / 64];
   float m = __builtin_nanf("");
   for (int64_t i = threadIdx.x; i < n; i += kFinish) m = nanmax(m, partials[i]);
-  lm[threadIdx.x] = m;
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = nanmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = m;
   __syncthreads();
-  for (int off = kFinish / 2; off > 0; off >>= 1) {
-    if (threadIdx.x < off) lm[threadIdx.x] = nanmax(lm[threadIdx.x], lm[threadIdx.x + off]);
-    __syncthreads();
+  if (threadIdx.x == 0) {
+    float t = lw[0];
+#pragma unroll
+    for (int w = 1; w < kFinish / 64; ++w) t = nanmax(t, lw[w]);
+    *out = t;
   }
-  if (threadIdx.x == 0) *out = lm[0];
 }
 
 // ----------------------------------------------------------------------------- full sum/max

@@ -168,8 +168,21 @@ std::vector<hipEvent_t> record_on_all_streams(cs_device *device) {
     CS_HIP(hipEventRecord(ev, s));
     evs.push_back(ev);
   };
-  record(nullptr);  // the null stream covers raw launches made outside library streams
-  for (cs_stream *s : device->streams) record(s->stream);
+  // A stream that is capturing a graph is skipped: an event recorded there would become a
+  // node of the graph, and capture starts on an idle stream (cs_graph_begin_capture waits
+  // for it), so none of its pending work can still use the block.
+  bool capturing = false;
+  std::vector<hipStream_t> live;
+  for (cs_stream *s : device->streams) {
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    CS_HIP(hipStreamIsCapturing(s->stream, &cap));
+    if (cap == hipStreamCaptureStatusNone) live.push_back(s->stream);
+    else capturing = true;
+  }
+  // The null stream covers raw launches made outside library streams; it is left alone
+  // while a capture is open.
+  if (!capturing) record(nullptr);
+  for (hipStream_t s : live) record(s);
   return evs;
 }
 
@@ -483,10 +496,12 @@ cs_status cs_stream_create(cs_device *device, cs_stream **stream) {
     if (!device) fail(CS_INVALID_DEVICE, "device is null");
     if (!stream) fail(CS_INVALID_VALUE, "stream is null");
     set_device(device);
+    hipStream_t hs = nullptr;
+    // Default priority on purpose: mixed-priority streams are not used anywhere here.
+    CS_HIP(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking));
     auto *s = new cs_stream();
     s->device = device;
-    // Default priority on purpose: mixed-priority streams are not used anywhere here.
-    CS_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+    s->stream = hs;
     std::lock_guard<std::recursive_mutex> l(device->mu);
     device->streams.insert(s);
     *stream = s;
@@ -875,6 +890,9 @@ cs_status cs_graph_begin_capture(cs_stream *stream) {
   return guard([&] {
     if (!stream) fail(CS_INVALID_VALUE, "stream is null");
     set_device(stream->device);
+    // Capture starts on an idle stream, so buffers released during the capture are not in
+    // use by work queued on it before (their release events skip capturing streams).
+    CS_HIP(hipStreamSynchronize(stream->stream));
     CS_HIP(hipStreamBeginCapture(stream->stream, hipStreamCaptureModeThreadLocal));
   });
 }

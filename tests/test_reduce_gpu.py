@@ -84,3 +84,43 @@ def test_rows_and_cols_ragged(gpu_tensors, shape):
     cs = t.colSum().flatArray()
     c64 = a.astype(np.float64).sum(axis=0)
     assert np.all(np.abs(cs - c64) <= 4 * fp64.EPS32 * np.abs(a).astype(np.float64).sum(axis=0) + 1e-30)
+
+
+@pytest.mark.parametrize("kind", ["sum", "max", "rows"])
+def test_combine_is_deterministic(gpu_tensors, kind):
+    """Block partials are combined in FP64 in a fixed order: 300 back-to-back launches (each
+    followed by a one-block reduction on the same stream) must all give the first launch's
+    bits and the FP64 oracle's value."""
+    T = gpu_tensors
+    rows, cols = 4096, 1024
+    a = inputs.heavy_tails(inputs.normal(rows * cols, 301), 302).reshape(rows, cols)
+    b = upload(T, a)
+    launches = 300
+    outs = alloc(T, launches)
+    outs64 = alloc(T, launches, np.float64)
+    small = upload(T, a[0, :100].copy())  # one block
+    small_out = alloc(T, launches)
+    f4, f8 = ctypes.sizeof(ctypes.c_float), ctypes.sizeof(ctypes.c_double)
+    for i in range(launches):
+        o, o64 = outs.ptr + i * f4, outs64.ptr + i * f8
+        if kind == "sum":
+            capi.check(capi.lib.cs_reduce_sum(T.stream, b.ptr, rows * cols, o, o64))
+        elif kind == "max":
+            capi.check(capi.lib.cs_reduce_max(T.stream, b.ptr, rows * cols, o))
+        else:
+            capi.check(capi.lib.cs_reduce_rows(T.stream, b.ptr, rows, cols, None, None, o, o64))
+        capi.check(capi.lib.cs_reduce_sum(T.stream, small.ptr, 100, small_out.ptr + i * f4, None))
+    got = download(T, outs, launches)
+    assert np.array_equal(got.view(np.uint32), np.full(launches, got[0], np.float32).view(np.uint32))
+    small_got = download(T, small_out, launches)
+    assert np.all(small_got == small_got[0])
+    assert abs(float(small_got[0]) - a[0, :100].astype(np.float64).sum()) <= fp64.sum_bound(a[0, :100]) * 2
+    if kind == "max":
+        assert got[0] == a.max()
+    else:
+        got64 = download(T, outs64, launches, np.float64)
+        assert np.all(got64 == got64[0])
+        exact = a.astype(np.float64).sum()
+        assert abs(got64[0] - exact) <= 1e-12 * np.abs(a).astype(np.float64).sum()
+    for h in (b, outs, outs64, small, small_out):
+        h.release()

@@ -161,3 +161,29 @@ def test_graph_replays_fused_and_reduction(gpu_tensors):
     tx.release()
     out.release()
     s.release()
+
+
+def test_release_during_capture(gpu_tensors):
+    """A buffer released while a stream captures a graph: its release events skip the
+    capturing stream (no event node enters the graph), later allocations that wait for
+    pending releases succeed, and the graph replays."""
+    T = gpu_tensors
+    n = 1 << 16
+    out = alloc(T, n)
+    junk = [alloc(T, 1 << 20) for _ in range(4)]
+    capi.check(lib.cs_graph_begin_capture(T.stream))
+    capi.check(lib.cs_fill_uniform(T.stream, out.ptr, n, 9, -1.0, 1.0))
+    for b in junk:
+        b.release()  # released mid-capture
+    g = ctypes.c_void_p()
+    capi.check(lib.cs_graph_end_capture(T.stream, ctypes.byref(g)))
+    big = [alloc(T, 64 << 20) for _ in range(4)]  # may wait on the pending releases
+    for _ in range(3):
+        capi.check(lib.cs_fill_uniform(T.stream, out.ptr, n, 1, 5.0, 6.0))
+        capi.check(lib.cs_graph_launch(T.stream, g))
+        T.synchronize()
+        x = download(T, out, n)
+        assert x.min() >= -1.0 and x.max() <= 1.0
+    capi.check(lib.cs_graph_release(g))
+    for b in big + [out]:
+        b.release()
