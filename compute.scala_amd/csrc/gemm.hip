@@ -23,9 +23,10 @@
 // (64 BF16 values, 32 hi/lo pairs or 128 e4m3 values); two stages live in LDS (128 KiB).
 // The next stage streams global -> LDS directly (buffer_load ... lds, no registers) while
 // the MFMAs of the current one run; inside a stage the fragments of the next MFMA step are
-// read from LDS while the current step's MFMAs issue. Each 128-byte LDS row stores its
-// 16-byte piece p at slot p ^ (row & 7), so the fragment reads of 8 consecutive rows hit
-// 8 different bank groups. FP32 keeps a register-staged single-buffer pipeline.
+// read from LDS while the current step's MFMAs issue. Each 128-byte LDS row R stores its
+// 16-byte piece p at slot p ^ ((R >> 1) & 7), so the fragment reads of 16 consecutive rows
+// (the lanes the LDS serves in one cycle) hit 16 different bank groups. FP32 keeps a
+// register-staged single-buffer pipeline.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -442,11 +443,13 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
   const int stages = Kp / P::kK;
 
   // Global -> LDS. One DMA instruction of a wave fills a 1 KiB chunk = 8 rows of a panel,
-  // lane l writing LDS bytes 16 l .. 16 l + 15: slot l & 7 of row l >> 3, which holds piece
-  // (l & 7) ^ (l >> 3) of that row. DMA d (0..15) of a stage: operand d & 1, chunk 4 (d >> 1)
-  // + wave, i.e. rows 32 (d >> 1) + 8 wave .. + 7.
+  // lane l writing LDS bytes 16 l .. 16 l + 15: slot l & 7 of row l >> 3 of the chunk. Row R
+  // of a panel stores its piece p at slot p ^ ((R >> 1) & 7). DMA d (0..15) of a stage:
+  // operand d & 1, chunk c = 4 (d >> 1) + wave, i.e. rows R = 8 c + (l >> 3), whose key
+  // (R >> 1) & 7 = 4 (wave & 1) + (l >> 4) is the same for all 8 DMAs of a lane.
   const int drow = lane >> 3;
-  const unsigned voff = unsigned((drow + 8 * wave) * rowb + (((lane & 7) ^ drow) << 4));
+  const int dkey = 4 * (wave & 1) + (drow >> 1);
+  const unsigned voff = unsigned((drow + 8 * wave) * rowb + (((lane & 7) ^ dkey) << 4));
   const __amdgpu_buffer_rsrc_t rx =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<char *>(X) + int64_t(m0) * rowb, 0, kTile * rowb, 0x00020000);
   const __amdgpu_buffer_rsrc_t ry =
@@ -458,9 +461,11 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
     __builtin_amdgcn_raw_ptr_buffer_load_lds((d & 1) ? ry : rx, (lds_void *)(l), 16, voff, soff, 0, 0);
   };
 
-  // Fragment reads: lane (half, l32) of tile i reads row wm*128 + 32 i + l32 (A) or
-  // wn*128 + 32 i + l32 (B); piece p sits at slot p ^ (row & 7) = p ^ (l32 & 7).
-  const int sw = l32 & 7;
+  // Fragment reads: lane (half, l32) of tile i reads row R = wm*128 + 32 i + l32 (A) or
+  // wn*128 + 32 i + l32 (B); piece p sits at slot p ^ ((R >> 1) & 7) = p ^ ((l32 >> 1) & 7).
+  // The LDS serves 16 lanes per cycle over 64 banks (two 128-byte rows): lanes l32 .. l32 + 15
+  // have 16 different (R & 1, (R >> 1) & 7), so they hit 16 different bank groups.
+  const int sw = (l32 >> 1) & 7;
   const int arow = (wm * 128 + l32) * kRowB, brow = kPanelB + (wn * 128 + l32) * kRowB;
   auto piece = [&](int buf, int row, int p) -> i32x4 {
     return *reinterpret_cast<const i32x4 *>(lds + buf * kStageB + row + ((p ^ sw) << 4));
