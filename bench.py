@@ -155,9 +155,19 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
     g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
     ms_hot = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_hot)), reps)
     us_event = per_launch_us(capi, T.stream, lambda i: launch(i % sets), 4 * sets)
+    # Reference point: the runtime's device-to-device copy of X into the output buffer (the
+    # same bytes in and out, the same rotating sets and graph timing) -- the streaming rate a
+    # kernel with no arithmetic reaches at this size.
+    g_copy = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_memcpy_dtod(T.stream, outs[i].ptr, xs[i].ptr, 4 * n))
+                                      for i in range(sets)])
+    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_copy)), reps_w)
+    ms_copy = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_copy)), reps)
+    lib.cs_graph_release(g_copy)
     bytes_per = 4 * n + 4 * 256 + 4 * n
     res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
                us_per_launch_events=us_event,
+               copy_reference={"bytes": 8 * n, "us": ms_copy * 1e3 / launches,
+                               "gbs": 8 * n * launches / (ms_copy * 1e-3) / 1e9},
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
     lib.cs_graph_release(g)
@@ -519,6 +529,7 @@ def main():
                                "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
                                "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
+                   "copy_reference": c2["copy_reference"],
                    "launch_config": c2["tuning"]},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": traffic,

@@ -59,6 +59,7 @@ class CsFusedDesc(ctypes.Structure):
 # cs_op
 OP_LITERAL, OP_PARAMETER, OP_TRANSFORM, OP_FILL, OP_EXTRACT = 1, 2, 3, 4, 5
 OP_NEG, OP_ABS, OP_SQRT, OP_TANH, OP_EXP, OP_LOG = 10, 11, 12, 13, 14, 15
+CHECK_EXP_BOUNDED = 114  # cs_math_check: exp where its argument cannot exceed 88
 OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD, OP_MIN, OP_MAX = 20, 21, 22, 23, 24, 25, 26
 
 MATH_PRECISE, MATH_FAST, MATH_ACCURATE = 0, 1, 2

@@ -445,6 +445,16 @@ class Emitter {
     return "*(" + gq_ + type + "*)(" + addr + ") = " + val + ";";
   }
 
+  // True when node `id` cannot exceed 88 (so exp of it cannot overflow): the negation of a
+  // value that is never negative (a square x*x of one node, or an absolute value). NaN
+  // inputs stay NaN either way.
+  bool bounded_above(int id) const {
+    const cs_node &n = dag_.nodes[static_cast<size_t>(id)];
+    if (n.op != CS_OP_NEG) return false;
+    const cs_node &m = dag_.nodes[static_cast<size_t>(n.a)];
+    return (m.op == CS_OP_MUL && m.a == m.b) || m.op == CS_OP_ABS;
+  }
+
   // extract(fill(x)) stands for x
   int resolve(int id) const {
     auto it = fill_of_.find(id);
@@ -497,7 +507,9 @@ class Emitter {
       case CS_OP_ABS: e = "fabsf(" + a + ")"; break;
       case CS_OP_SQRT: e = "sqrtf(" + a + ")"; break;
       case CS_OP_TANH: e = (acc ? "csmi_tanhf(" : "csmi_tanhf_p(") + a + ")"; break;
-      case CS_OP_EXP: e = (fast ? "__expf(" : acc ? "csmi_expf(" : "expf(") + a + ")"; break;
+      case CS_OP_EXP:
+        e = (fast ? "__expf(" : acc ? "csmi_expf(" : bounded_above(n.a) ? "csmi_expf_np(" : "expf(") + a + ")";
+        break;
       case CS_OP_LOG: e = (fast ? "__logf(" : "logf(") + a + ")"; break;
       case CS_OP_ADD: e = a + " + " + b; break;
       case CS_OP_SUB: e = a + " - " + b; break;

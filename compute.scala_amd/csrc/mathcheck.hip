@@ -25,6 +25,7 @@ __device__ __forceinline__ int64_t ordered(float f) {
 __device__ float eval32(int op, int mode, float x) {
   switch (op) {
     case CS_OP_EXP: return mode == CS_MATH_ACCURATE ? csmi_expf(x) : mode == CS_MATH_FAST ? __expf(x) : expf(x);
+    case CS_CHECK_EXP_BOUNDED: return csmi_expf_np(x);
     case CS_OP_TANH: return mode == CS_MATH_ACCURATE ? csmi_tanhf(x) : csmi_tanhf_p(x);
     case CS_OP_LOG: return mode == CS_MATH_FAST ? __logf(x) : logf(x);
     case CS_OP_SQRT: return sqrtf(x);
@@ -34,7 +35,8 @@ __device__ float eval32(int op, int mode, float x) {
 
 __device__ double eval64(int op, double x) {
   switch (op) {
-    case CS_OP_EXP: return exp(x);
+    case CS_OP_EXP:
+    case CS_CHECK_EXP_BOUNDED: return exp(x);
     case CS_OP_TANH: return tanh(x);
     case CS_OP_LOG: return log(x);
     case CS_OP_SQRT: return sqrt(x);
@@ -82,7 +84,7 @@ extern "C" cs_status cs_math_check(cs_stream *stream, int op, int mode, uint32_t
                                    unsigned long long *stats_device) {
   return guard([&] {
     if (!stream || !stats_device) fail(CS_INVALID_VALUE, "null argument");
-    if (op != CS_OP_EXP && op != CS_OP_TANH && op != CS_OP_LOG && op != CS_OP_SQRT)
+    if (op != CS_OP_EXP && op != CS_OP_TANH && op != CS_OP_LOG && op != CS_OP_SQRT && op != CS_CHECK_EXP_BOUNDED)
       fail(CS_INVALID_VALUE, "unsupported op");
     if (mode != CS_MATH_PRECISE && mode != CS_MATH_FAST && mode != CS_MATH_ACCURATE)
       fail(CS_INVALID_VALUE, "unknown math mode");

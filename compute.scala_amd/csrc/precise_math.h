@@ -5,12 +5,13 @@
 // Precise mode (each result within 2 ulp, checked over every FP32 input on the device):
 //   csmi_tanhf_p(x), branch free: for |x| < 0.625 the odd polynomial x + x^3 P(x^2), P a
 //            degree-4 weighted least-squares fit of (tanh(x) - x) / x^3 (about 0.86 ulp in
-//            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} = 2^(th + tl) from the
-//            extended-precision argument th + tl = 2|x| log2(e) (log2 e as a float pair):
-//            E = 2^th (1 + tl ln 2) with the hardware 2^th (th in [1.8, 27.4], normal results).
-//            |x| is clamped to 9.5 (tanh rounds to 1 beyond 9.01) by a compare that keeps NaN,
-//            so no range checks are needed.
-//   exp stays the device library's expf, which is already branch free.
+//            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} from the hardware 2^t
+//            on t = 2|x| log2(e) in [1.8, 27.4] (normal results). The error of E reaches the
+//            result scaled by 2/(E + 1)^2 * E, small enough that the whole function stays
+//            within 2 ulp. |x| is clamped to 9.5 (tanh rounds to 1 beyond 9.01) by a compare
+//            that keeps NaN, so no range checks are needed.
+//   exp is the device library's expf (branch free, 1 ulp), or csmi_expf_np where the
+//            argument cannot overflow (the same steps without the overflow check).
 //
 // Accurate mode (each result within about 0.7 ulp, checked likewise), FMA-rate only:
 //   exp(x) = 2^n (1 + q),  n = rint(x log2 e),  r = x - n ln2 (Cody-Waite, ln2 in two
@@ -24,13 +25,11 @@
 #pragma once
 
 __device__ __forceinline__ float csmi_tanhf_p(float x) {
+#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
   const float ax = fabsf(x);
   const float a = !(ax > 9.5f) ? ax : 9.5f;  // keeps NaN; tanh rounds to 1 beyond 9.01
-  // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) = th + tl in [1.8, 27.4]
-  const float th = a * 2.88539004f;                                        // 2 log2(e), high part
-  const float tl = fmaf(a, 3.85192598e-08f, fmaf(a, 2.88539004f, -th));  // + low part
-  const float e0 = __builtin_amdgcn_exp2f(th);
-  const float e = fmaf(e0, tl * 0.693147182f, e0);  // 2^(th + tl), first order in tl (|tl| < 2^-19)
+  // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) in [1.8, 27.4]
+  const float e = __builtin_amdgcn_exp2f(a * 2.88539004f);
   const float big = fmaf(__builtin_amdgcn_rcpf(e + 1.0f), -2.0f, 1.0f);
   // |x| < 0.625: a + a^3 P(a^2)
   const float s = a * a;
@@ -43,7 +42,22 @@ __device__ __forceinline__ float csmi_tanhf_p(float x) {
   return copysignf(a < 0.625f ? small : big, x);  // NaN: a is NaN, so is big
 }
 
+// exp for an argument that cannot exceed 88 (the generated code uses it where the argument
+// is the negation of a square or of an absolute value): 2^n 2^f from the extended-precision
+// argument x log2(e) = ph + pl (log2 e as a float pair), n = rint(ph), the hardware 2^f on
+// |f| <= 1/2 and an exact scaling that produces subnormal results; below -104 the result
+// rounds to +0 (as does -inf, for which ph - n would be NaN). No overflow check is needed.
+__device__ __forceinline__ float csmi_expf_np(float x) {
+#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+  const float ph = x * 1.44269502f;
+  const float pl = fmaf(x, 1.92596299e-08f, fmaf(x, 1.44269502f, -ph));
+  const float n = rintf(ph);
+  const float r = ldexpf(__builtin_amdgcn_exp2f((ph - n) + pl), static_cast<int>(n));
+  return x < -104.0f ? 0.0f : r;  // NaN fails the compare and r is NaN
+}
+
 __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, float &ql) {
+#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
   n = rintf(x * 1.44269502f);
   const float r1 = fmaf(n, -0.693145751953125f, x);  // n * L1 is exact (L1 has 15 bits)
   const float rh = fmaf(n, -1.42860677e-06f, r1);
@@ -62,6 +76,7 @@ __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, fl
 // Branch free: the argument is clamped to [-104, 89] (exp(89) overflows to +inf and
 // exp(-104) rounds to +0, as the unclamped values would), and NaN is selected at the end.
 __device__ __forceinline__ float csmi_expf(float x) {
+#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
   const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
   float n, qh, ql;
   csmi_exp_reduce(xc, n, qh, ql);
@@ -72,6 +87,7 @@ __device__ __forceinline__ float csmi_expf(float x) {
 }
 
 __device__ __forceinline__ float csmi_tanhf(float x) {
+#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
   const float a = fminf(fabsf(x), 9.5f);  // tanh rounds to 1 beyond 9.01; NaN selected below
   float n, qh, ql;
   csmi_exp_reduce(2.0f * a, n, qh, ql);

@@ -359,11 +359,14 @@ cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K,
 
 /* ---------------------------------------------------------------- accuracy check
  * For every FP32 bit pattern in [start, start + count), compares the elementwise function
- * `op` (CS_OP_EXP, CS_OP_TANH, CS_OP_LOG or CS_OP_SQRT, as math mode `mode` computes it in
+ * `op` (CS_OP_EXP, CS_OP_TANH, CS_OP_LOG, CS_OP_SQRT or CS_CHECK_EXP_BOUNDED, as math mode `mode` computes it in
  * generated kernels) with its FP64 counterpart rounded to FP32. stats_device points at 3 zero-initialised
  * device uint64 values that receive max(ulp distance), the count of inputs over 1 ulp and
  * the bits of the worst input. The reference has no counterpart: its OpenCL runtime's
  * math library is not testable from its API. */
+/* For cs_math_check only: exp as the precise mode generates it where its argument cannot
+ * exceed 88 (the negation of a square or of an absolute value); meaningful for x <= 0. */
+#define CS_CHECK_EXP_BOUNDED 114
 cs_status cs_math_check(cs_stream *stream, int op, int mode, uint32_t start, uint32_t count,
                         unsigned long long *stats_device);
 

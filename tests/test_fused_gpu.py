@@ -69,13 +69,14 @@ def test_math_functions_over_every_float(gpu_tensors, mode):
     stats = alloc(T, 3, np.uint64)
     m = {"precise": capi.MATH_PRECISE, "accurate": capi.MATH_ACCURATE}[mode]
     if mode == "precise":
-        limits = {capi.OP_EXP: 2, capi.OP_TANH: 2, capi.OP_LOG: 2, capi.OP_SQRT: 0}
+        # the bounded exp (generated where the argument cannot exceed 88) over every x <= 0
+        limits = {capi.OP_EXP: 2, capi.OP_TANH: 2, capi.OP_LOG: 2, capi.OP_SQRT: 0, capi.CHECK_EXP_BOUNDED: 2}
     else:
         limits = {capi.OP_EXP: 1, capi.OP_TANH: 1}
     for op, limit in limits.items():
         zero = np.zeros(3, np.uint64)
         capi.check(capi.lib.cs_memcpy_htod(T.stream, ctypes.c_void_p(stats.ptr), zero.ctypes.data_as(ctypes.c_void_p), 24))
-        for half in range(2):
+        for half in ((1,) if op == capi.CHECK_EXP_BOUNDED else (0, 1)):
             capi.check(capi.lib.cs_math_check(T.stream, op, m, half << 31, 1 << 31, stats.ptr))
         s = download(T, stats, 3, np.uint64)
         x = np.array([s[2]], np.uint32).view(np.float32)[0]
