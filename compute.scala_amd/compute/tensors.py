@@ -560,6 +560,53 @@ class _GemmTensor(NonInlineTensor):
             b.release()
 
 
+def _split_matrix(n, d, index):
+    """The view of slice `index` along dimension d of a rank-n tensor: n source rows x
+    (n - 1 + 1) columns (Tensors.scala:1038-1073)."""
+    m = [0.0] * (n * n)
+    for i in range(d):
+        m[i * n + i] = 1.0
+    m[d * n + n - 1] = float(index)
+    for i in range(d + 1, n):
+        m[i * n + i - 1] = 1.0
+    return tuple(m)
+
+
+def _split_fold(closure, shape):
+    """Recognises `t.split(d).reduce(_ + _)` and `t.split(d).reduce(Tensor.max)` over a
+    materialised tensor t, the way the reference writes axis sums and maxima (README.md:342,
+    TensorsSpec.scala:477; a left fold of elementwise ops over the slices, in order).
+    Returns (parameter, d, op) or None."""
+    if not isinstance(closure, trees.Binary) or closure.op not in ("+", "max"):
+        return None
+    op = closure.op
+    slices = []
+    node = closure
+    while isinstance(node, trees.Binary) and node.op == op:
+        slices.append(node.rhs)
+        node = node.lhs
+    slices.append(node)
+    slices.reverse()
+    param = None
+    for s in slices:
+        if not (isinstance(s, trees.Extract) and isinstance(s.array, trees.Transform) and
+                isinstance(s.array.array, trees.ArrayParameter)):
+            return None
+        if param is None:
+            param = s.array.array
+        elif s.array.array is not param:
+            return None
+    n = len(param.shape)
+    if n < 1 or len(shape) != n - 1:
+        return None
+    for d in range(n):
+        if param.shape[d] != len(slices) or tuple(param.shape[:d] + param.shape[d + 1:]) != tuple(shape):
+            continue
+        if all(s.array.matrix == _split_matrix(n, d, i) for i, s in enumerate(slices)):
+            return param, d, op
+    return None
+
+
 class TensorSeq(Sequence):
     """The sequence returned by split (Tensors.scala:1038-1073)."""
 
@@ -580,15 +627,7 @@ class TensorSeq(Sequence):
             index += len(self)
         if not 0 <= index < len(self):
             raise IndexError(index)
-        n = len(self._t.shape)
-        d = self._d
-        m = [0.0] * (n * n)  # n source rows x (n - 1 + 1) columns
-        for i in range(d):
-            m[i * n + i] = 1.0
-        m[d * n + n - 1] = float(index)
-        for i in range(d + 1, n):
-            m[i * n + i - 1] = 1.0
-        return self._t._transform(self._new_shape, m)
+        return self._t._transform(self._new_shape, _split_matrix(len(self._t.shape), self._d, index))
 
     def __str__(self):
         return "TensorSeq(" + ", ".join(t.toString() for t in self) + ")"
@@ -702,8 +741,11 @@ class Tensors(object):
     Mirror of `trait Tensors` plus the plug-in wiring of `object gpu` (gpu.scala:15-27)."""
 
     def __init__(self, ordinal=0, math_mode=capi.MATH_PRECISE, vector_width=0, items_per_thread=0,
-                 block_size=0, offline=False):
+                 block_size=0, offline=False, axis_reductions=True):
         self.options = capi.CsFusedOptions(math_mode, vector_width, items_per_thread, block_size, 0)
+        # split(d).reduce(+ or max) folds run as one axis reduction (compensated FP32 sums
+        # combined in FP64, exact maxima) instead of one kernel with a term per slice
+        self.axis_reductions = axis_reductions
         self.Tensor = TensorCompanion(self)
         self._lock = threading.RLock()
         self.launches = 0
@@ -821,6 +863,12 @@ class Tensors(object):
     def _enqueue_closure(self, closure, shape, memo):
         """enqueueClosure (Tensors.scala:1291-1392): compile or fetch, materialise the leaves,
         allocate the output, launch."""
+        if self.axis_reductions:
+            fold = _split_fold(closure, shape)
+            if fold is not None:
+                out = self._axis_reduce(fold, memo)
+                if out is not None:
+                    return out
         f, leaves, nout = self.compile(closure, shape)
         try:
             args = [leaf.id._buffer(memo) for leaf in leaves]
@@ -835,6 +883,35 @@ class Tensors(object):
                     a.release()
         finally:
             lib.cs_fused_release(f)
+
+
+    def _axis_reduce(self, fold, memo):
+        """A recognised split fold over dimension d of a dense [outer, n, inner] buffer: row
+        kernels when d is innermost, column sums otherwise (max only along the innermost
+        dimension, up to 64 column passes). None: not covered, the fold is compiled."""
+        param, d, op = fold
+        outer = _product(param.shape[:d])
+        n = param.shape[d]
+        inner = _product(param.shape[d + 1:])
+        if inner != 1 and (op == "max" or outer > 64):
+            return None
+        src = param.id._buffer(memo)
+        try:
+            out = self._alloc(max(outer * inner, 1) * 4)
+            if outer * inner == 0:
+                return out
+            if inner == 1:
+                sums, maxima = (out.ptr, None) if op == "+" else (None, out.ptr)
+                check(lib.cs_reduce_rows(self._stream, src.ptr, outer, n, sums, maxima, None, None),
+                      "cs_reduce_rows")
+            else:
+                for o in range(outer):
+                    check(lib.cs_reduce_cols(self._stream, src.ptr + o * n * inner * 4, n, inner,
+                                             out.ptr + o * inner * 4), "cs_reduce_cols")
+            self.launches += 1
+            return out
+        finally:
+            src.release()
 
 
 _gpu = None

@@ -392,6 +392,7 @@ class Emitter {
     const int nout = static_cast<int>(dag_.outputs.size());
     uniform_.clear();
     fill_of_.clear();
+    chunks_.clear();
     s << "  const unsigned vbase = (" << tile << ") * (" << bdim_ << " * " << U << "u) + threadIdx.x;\n";
     // phase 0: vector index, validity and coordinates of element 0 of every vector
     s << "  unsigned q[" << U << "]; bool live[" << U << "];\n";
@@ -558,6 +559,7 @@ class Emitter {
       bool upper_only = false;  // bare parameter: only idx < dim is checked
       int64_t shift = 0;        // integral constant term
       bool others_by4 = true;   // integral coefficients of the other coordinates, all multiples of 4
+      std::string unshifted;    // expr without the integral constant term
     };
     std::vector<Row> rows(static_cast<size_t>(src_rank));
     if (!ar.has_view) {
@@ -597,9 +599,12 @@ class Emitter {
         if (terms.empty()) continue;
         row.empty = false;
         row.fractional = frac;
-        std::string e;
+        std::string e, u;
         for (size_t t = 0; t < terms.size(); ++t) e += (t ? " + " : "") + terms[t];
+        const size_t nu = terms.size() - (row.shift != 0 ? 1 : 0);  // the constant term is last
+        for (size_t t = 0; t < nu; ++t) u += (t ? " + " : "") + terms[t];
         row.expr = e;
+        row.unshifted = u;
       }
     }
     std::vector<int64_t> stride(static_cast<size_t>(src_rank), 1);
@@ -656,6 +661,38 @@ class Emitter {
       s << "  }\n";
       return;
     }
+    // A row read along the innermost dimension, possibly shifted by a constant k (a translate):
+    // element 0 of a vector sits at a fixed phase (k mod 4) inside a 16-byte chunk of the
+    // source row, so the V elements come from V/4 (+1) aligned chunks, each entirely inside or
+    // outside the row (the row length is a multiple of 4). Out-of-row chunks load from the
+    // leaf's base instead and are replaced by the padding: no per-element branches. Gathers of
+    // one leaf that differ only in k share their chunks (a value and its translate read the
+    // same row).
+    const bool window = contiguous && V % 4 == 0 && p.shape[moving] % 4 == 0 && rows[moving].others_by4 &&
+                        !rows[moving].upper_only;
+    std::vector<std::string> chunk(static_cast<size_t>(V / 4 + 1));
+    std::vector<bool> chunk_new(chunk.size(), false);
+    int phase = 0, nch = 0;
+    if (window) {
+      const Row &mv = rows[moving];
+      phase = static_cast<int>(((mv.shift % 4) + 4) % 4);
+      nch = V / 4 + (phase ? 1 : 0);
+      std::string key = ptr;
+      for (int y = 0; y < src_rank; ++y) key += "|" + (rows[y].empty ? std::string("_") : y == moving ? rows[y].unshifted : rows[y].expr);
+      for (int q = 0; q < nch; ++q) {
+        // chunk q starts at 4q - phase from element 0, i.e. at 4q - phase + k from the unshifted index
+        const std::string ck = key + "|" + std::to_string(4 * q - phase + mv.shift);
+        auto it = chunks_.find(ck);
+        if (it != chunks_.end()) {
+          chunk[q] = it->second;
+        } else {
+          chunk[q] = g + "_w" + std::to_string(q);
+          chunk_new[q] = true;
+          chunks_[ck] = chunk[q];
+          s << "  csmi_f4 " << chunk[q] << "[" << U << "];\n";
+        }
+      }
+    }
     s << "  float " << name << "[" << U << "][" << V << "];\n";
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
     if (contiguous) {
@@ -677,30 +714,22 @@ class Emitter {
       for (size_t c = 0; c < conds.size(); ++c) others += (c ? " && " : "") + conds[c];
       s << "      const bool " << g << "_ok = " << others << ";\n";
       s << "      const " << gq_ << "float* " << g << "_p = " << ptr << " + (" << oc.first << ");\n";
-      // A row read shifted by a constant along the innermost dimension (a translate): element 0
-      // of a vector sits at a fixed phase (shift mod 4) inside a 16-byte chunk of the source
-      // row, so the V elements come from V/4 (+1) aligned chunks, each entirely inside or
-      // outside the row (the row length is a multiple of 4). Out-of-row chunks load from the
-      // leaf's base instead and are replaced by the padding: no per-element branches.
-      const Row &mv = rows[moving];
-      if (mv.shift != 0 && V % 4 == 0 && p.shape[moving] % 4 == 0 && mv.others_by4 && !mv.upper_only) {
-        const int phase = static_cast<int>(((mv.shift % 4) + 4) % 4);
-        const int nch = V / 4 + (phase ? 1 : 0);
+      if (window) {
         for (int q = 0; q < nch; ++q) {
+          if (!chunk_new[q]) continue;
           const std::string cs = g + "_s" + std::to_string(q), ck = g + "_k" + std::to_string(q);
-          const std::string w = g + "_w" + std::to_string(q);
+          const std::string w = chunk[q];
           const int off = 4 * q - phase;
           s << "      const int " << cs << " = " << yi << " + " << off << ";\n";
           s << "      const bool " << ck << " = " << g << "_ok && " << cs << " >= 0 && " << cs << " < "
             << p.shape[moving] << ";\n";
           s << "      const csmi_f4 " << w << "l = *(const " << gq_ << "csmi_f4*)(" << ck << " ? " << g << "_p + "
             << off << " : " << ptr << ");\n";
-          s << "      const csmi_f4 " << w << " = " << ck << " ? " << w << "l : csmi_f4{" << pad << ", " << pad << ", "
-            << pad << ", " << pad << "};\n";
+          s << "      " << w << "[u] = " << ck << " ? " << w << "l : csmi_f4{" << pad << ", " << pad << ", " << pad
+            << ", " << pad << "};\n";
         }
         for (int v = 0; v < V; ++v)
-          s << "      " << name << "[u][" << v << "] = " << g << "_w" << (phase + v) / 4 << "[" << (phase + v) % 4
-            << "];\n";
+          s << "      " << name << "[u][" << v << "] = " << chunk[(phase + v) / 4] << "[u][" << (phase + v) % 4 << "];\n";
         s << "  }\n";
         return;
       }
@@ -754,6 +783,7 @@ class Emitter {
   std::ostringstream *body_ = nullptr;
   std::set<int> uniform_;
   std::map<int, int> fill_of_;
+  std::map<std::string, std::string> chunks_;  // aligned source chunks already loaded, by position
 };
 
 std::vector<std::string> math_options(int mode) {

@@ -19,14 +19,13 @@
 // threads (4 wave64s in 2x2) per 256x256 output tile, each wave owning 128x128 as 4x4 MFMA
 // tiles of 32x32 (256 FP32 accumulators per lane).
 //
-// BF16, BF16X3 and FP8 share one pipeline. A K stage is 128 bytes of every operand row
-// (64 BF16 values, 32 hi/lo pairs or 128 e4m3 values); two stages live in LDS (128 KiB).
+// All variants share one pipeline. A K stage is 128 bytes of every operand row (32 FP32
+// values, 64 BF16 values, 32 hi/lo pairs or 128 e4m3 values); two stages live in LDS (128 KiB).
 // The next stage streams global -> LDS directly (buffer_load ... lds, no registers) while
 // the MFMAs of the current one run; inside a stage the fragments of the next MFMA step are
 // read from LDS while the current step's MFMAs issue. Each 128-byte LDS row R stores its
 // 16-byte piece p at slot p ^ ((R >> 1) & 7), so the fragment reads of 16 consecutive rows
-// (the lanes the LDS serves in one cycle) hit 16 different bank groups. FP32 keeps a
-// register-staged single-buffer pipeline.
+// (the lanes the LDS serves in one cycle) hit 16 different bank groups.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -288,93 +287,7 @@ __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__r
   }
 }
 
-// ---------------------------------------------------------------------------- FP32 kernel
-constexpr int kF32StageK = 16;                      // K values per stage
-constexpr int kF32RowBytes = kF32StageK * 4;        // 64
-constexpr int kF32Stride = kF32RowBytes + 16;       // padded LDS row
-constexpr int kF32Chunks = kTile * kF32RowBytes / 16 / kThreads;  // 16-byte loads per thread per operand
-
-// X: [Mp][Kp] FP32, Y: [Np][Kp] FP32 (Kp a multiple of 16).
-__global__ __launch_bounds__(kThreads, 1) void gemm_f32_kernel(int M, int N, int Kp, int tiles_m, int tiles_n,
-                                                               const float *__restrict__ X,
-                                                               const float *__restrict__ Y, float *__restrict__ C,
-                                                               int ldc) {
-  __shared__ __attribute__((aligned(16))) char lds[2 * kTile * kF32Stride];
-  char *xs = lds, *ys = lds + kTile * kF32Stride;
-  const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const int wm = wave / 2, wn = wave % 2;
-  const int half = lane / 32, l32 = lane % 32;
-  int tm, tn;
-  tile_coords(blockIdx.x, tiles_m, tiles_n, tm, tn);
-  const int m0 = tm * kTile, n0 = tn * kTile;
-  const char *xg = reinterpret_cast<const char *>(X), *yg = reinterpret_cast<const char *>(Y);
-  const int64_t kpb = int64_t(Kp) * 4;
-  i32x4 rx[kF32Chunks], ry[kF32Chunks];
-  auto load_stage = [&](int ks) {
-#pragma unroll
-    for (int c = 0; c < kF32Chunks; ++c) {
-      const int idx = threadIdx.x + c * kThreads;
-      const int row = idx / (kF32RowBytes / 16), col = idx % (kF32RowBytes / 16);
-      const int64_t go = int64_t(ks) * kF32RowBytes + col * 16;
-      rx[c] = *reinterpret_cast<const i32x4 *>(xg + (m0 + row) * kpb + go);
-      ry[c] = *reinterpret_cast<const i32x4 *>(yg + (n0 + row) * kpb + go);
-    }
-  };
-  auto store_stage = [&]() {
-#pragma unroll
-    for (int c = 0; c < kF32Chunks; ++c) {
-      const int idx = threadIdx.x + c * kThreads;
-      const int row = idx / (kF32RowBytes / 16), col = idx % (kF32RowBytes / 16);
-      *reinterpret_cast<i32x4 *>(xs + row * kF32Stride + col * 16) = rx[c];
-      *reinterpret_cast<i32x4 *>(ys + row * kF32Stride + col * 16) = ry[c];
-    }
-  };
-  f32x16 acc[kWaveTiles][kWaveTiles];
-#pragma unroll
-  for (int i = 0; i < kWaveTiles; ++i)
-#pragma unroll
-    for (int j = 0; j < kWaveTiles; ++j)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) acc[i][j][e] = 0.0f;
-  const int stages = Kp / kF32StageK;
-  load_stage(0);
-  for (int ks = 0; ks < stages; ++ks) {
-    store_stage();
-    __syncthreads();
-    if (ks + 1 < stages) load_stage(ks + 1);
-    // instruction s of the stage: lane half h reads K value 8h + s (a K permutation that
-    // A and B share, so the sum is unchanged)
-#pragma unroll
-    for (int s = 0; s < kF32StageK / 2; ++s) {
-      const int off = half * (kF32RowBytes / 2) + s * 4;
-      float fa[kWaveTiles], fb[kWaveTiles];
-#pragma unroll
-      for (int i = 0; i < kWaveTiles; ++i) {
-        fa[i] = *reinterpret_cast<const float *>(xs + (wm * 128 + i * 32 + l32) * kF32Stride + off);
-        fb[i] = *reinterpret_cast<const float *>(ys + (wn * 128 + i * 32 + l32) * kF32Stride + off);
-      }
-#pragma unroll
-      for (int i = 0; i < kWaveTiles; ++i)
-#pragma unroll
-        for (int j = 0; j < kWaveTiles; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(fa[i], fb[j], acc[i][j], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  // acc[i][j][e]: row = 8*(e/4) + 4*half + e%4, col = l32 of the 32x32 tile (i, j)
-#pragma unroll
-  for (int i = 0; i < kWaveTiles; ++i)
-#pragma unroll
-    for (int j = 0; j < kWaveTiles; ++j)
-#pragma unroll
-      for (int e = 0; e < 16; ++e) {
-        const int r = m0 + wm * 128 + i * 32 + 8 * (e / 4) + 4 * half + (e % 4);
-        const int c = n0 + wn * 128 + j * 32 + l32;
-        if (r < M && c < N) C[int64_t(r) * ldc + c] = acc[i][j][e];
-      }
-}
-
-// ------------------------------------------------------------- BF16 / BF16X3 / FP8 kernel
+// ------------------------------------------------------------------------ main kernel
 constexpr int kRowB = 128;                          // bytes of one operand row per K stage
 constexpr int kPanelB = kTile * kRowB;              // 32 KiB: one operand's rows of a stage
 constexpr int kStageB = 2 * kPanelB;                // X and Y panels
@@ -382,6 +295,12 @@ constexpr size_t kLdsBytes = size_t(2) * kStageB;   // two stages in flight: 128
 
 template <int KIND>
 struct Pipe;
+template <>
+struct Pipe<kF32> {
+  // 4 fragment steps of 8 K values; each fragment (4 FP32 values per lane) feeds 4 MFMAs of K 2
+  static constexpr int kBytesPerK = 4, kK = 32, kSteps = 4;
+  typedef i32x4 Frag;
+};
 template <>
 struct Pipe<kBF16> {
   static constexpr int kBytesPerK = 2, kK = 64, kSteps = 4;  // 4 MFMA K steps of 16
@@ -472,12 +391,16 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
   };
   // BF16 step s uses piece 2 s + half (K 16 s + 8 half .. + 7, the MFMA's own K order);
   // BF16X3 step s: high part piece 2 s + half, low part 4 + 2 s + half; FP8 step s: pieces
-  // 4 s + 2 half and 4 s + 2 half + 1 (K 64 s + 32 half .. + 31).
+  // 4 s + 2 half and 4 s + 2 half + 1 (K 64 s + 32 half .. + 31). FP32 step s: piece
+  // 2 s + half, K 8 s + 4 half + t for t = 0 .. 3; MFMA t of the step takes value t, so it
+  // sums K 8 s + t and 8 s + 4 + t (a K order that A and B share: the sum is unchanged).
   auto frag = [&](int buf, int row, int s, int part) -> F {
     if constexpr (KIND == kFP8) {
       const int p0 = 4 * s + 2 * half;
       const i32x4 x = piece(buf, row, p0), y = piece(buf, row, p0 + 1);
       return i32x8{x[0], x[1], x[2], x[3], y[0], y[1], y[2], y[3]};
+    } else if constexpr (KIND == kF32) {
+      return piece(buf, row, 2 * s + half);
     } else {
       return __builtin_bit_cast(bf16x8, piece(buf, row, 4 * part + 2 * s + half));
     }
@@ -620,13 +543,32 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
           }
         }
         CSMI_SCHED_FENCE();
+        if constexpr (KIND == kF32) {
+          // 4 MFMAs per tile, value t of every fragment in round t: 16 independent
+          // accumulators between two uses of one. (__int_as_float, not __builtin_bit_cast: the
+          // latter applied to a vector element reads element 0 with this compiler.)
 #pragma unroll
-        for (int g = 0; g < kWaveTiles; ++g) {
+          for (int t = 0; t < 4; ++t) {
 #pragma unroll
-          for (int j = 0; j < kWaveTiles; ++j)
-            mma<KIND>(fa[s & 1][g], fb[s & 1][j], acc[g][j], sa[0][g], sb[0][j]);
-          if (kMore) dma_slot(s * kWaveTiles + g, ks + 1, buf ^ 1);
-          CSMI_SCHED_FENCE();
+            for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+              for (int j = 0; j < kWaveTiles; ++j)
+                acc[g][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(__int_as_float(fa[s & 1][g][t]),
+                                                                 __int_as_float(fb[s & 1][j][t]),
+                                                                 acc[g][j], 0, 0, 0);
+              if (kMore && t == 0) dma_slot(s * kWaveTiles + g, ks + 1, buf ^ 1);
+              CSMI_SCHED_FENCE();
+            }
+          }
+        } else {
+#pragma unroll
+          for (int g = 0; g < kWaveTiles; ++g) {
+#pragma unroll
+            for (int j = 0; j < kWaveTiles; ++j)
+              mma<KIND>(fa[s & 1][g], fb[s & 1][j], acc[g][j], sa[0][g], sb[0][j]);
+            if (kMore) dma_slot(s * kWaveTiles + g, ks + 1, buf ^ 1);
+            CSMI_SCHED_FENCE();
+          }
         }
       }
       if (kMore) {
@@ -655,25 +597,6 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
-void run_f32(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
-             int ldc) {
-  const int64_t Mp = round_up(M, kTile), Np = round_up(N, kTile), Kp = round_up(K, kF32StageK);
-  const int64_t xbytes = Mp * Kp * 4, ybytes = Np * Kp * 4;
-  char *base = static_cast<char *>(stream_scratch(stream, static_cast<size_t>(xbytes + ybytes + 4096)));
-  float *x = reinterpret_cast<float *>(base), *y = reinterpret_cast<float *>(base + xbytes);
-  hipStream_t s = stream->stream;
-  const int vec_a = (lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0) ? 1 : 0;
-  const int vec_b = (ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) ? 1 : 0;
-  hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
-                     lda, int(Mp), int(Kp), vec_a, x);
-  hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256), 0, s, B,
-                     N, K, ldb, int(Np), int(Kp), vec_b, y);
-  const int tiles_m = int(Mp / kTile), tiles_n = int(Np / kTile);
-  hipLaunchKernelGGL(gemm_f32_kernel, dim3(unsigned(tiles_m * tiles_n)), dim3(kThreads), 0, s, M, N, int(Kp),
-                     tiles_m, tiles_n, x, y, C, ldc);
-  CS_HIP(hipGetLastError());
-}
-
 template <int KIND>
 void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
              int ldc) {
@@ -699,6 +622,11 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
                        K, lda, int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x), sx);
     hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Kp / 128), unsigned(Np / 64)), dim3(256), 0, s, B, N,
                        K, ldb, int(Np), int(Kp), reinterpret_cast<uint8_t *>(y), sy);
+  } else if (KIND == kF32) {
+    hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
+                       lda, int(Mp), int(Kp), vec_a, reinterpret_cast<float *>(x));
+    hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256), 0, s, B,
+                       N, K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<float *>(y));
   } else {
     constexpr bool kSplit = KIND == kBF16X3;
     hipLaunchKernelGGL(pack_rows_bf16_kernel<kSplit>, dim3(unsigned((Mp * (Kp / 8) + 255) / 256)), dim3(256), 0, s,
@@ -738,7 +666,7 @@ extern "C" cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K
     set_device(stream->device);
     const int Ke = K == 0 ? 1 : K;  // K = 0: a zero-padded stage gives C = 0
     switch (variant) {
-      case CS_GEMM_FP32: run_f32(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_FP32: run_lds<kF32>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
       case CS_GEMM_BF16: run_lds<kBF16>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
       case CS_GEMM_BF16X3: run_lds<kBF16X3>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
       case CS_GEMM_FP8: run_lds<kFP8>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;

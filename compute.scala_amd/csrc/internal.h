@@ -83,6 +83,8 @@ class Arena {
   ~Arena();
   Block allocate(size_t bytes);
   void release(Block b, std::vector<hipEvent_t> events);
+  // Waits for every pending free and hands its block back; the release events are destroyed.
+  void settle() { drain_pending(true); }
   void trim();
   size_t reserved() const { return reserved_; }
   size_t in_use() const { return in_use_; }

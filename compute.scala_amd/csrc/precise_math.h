@@ -6,10 +6,9 @@
 //   csmi_tanhf_p(x), branch free: for |x| < 0.625 the odd polynomial x + x^3 P(x^2), P a
 //            degree-4 weighted least-squares fit of (tanh(x) - x) / x^3 (about 0.86 ulp in
 //            FP32 arithmetic); above, 1 - 2 / (E + 1) with E = e^{2|x|} from the hardware 2^t
-//            on t = 2|x| log2(e) in [1.8, 27.4] (normal results). The error of E reaches the
-//            result scaled by 2/(E + 1)^2 * E, small enough that the whole function stays
-//            within 2 ulp. |x| is clamped to 9.5 (tanh rounds to 1 beyond 9.01) by a compare
-//            that keeps NaN, so no range checks are needed.
+//            on t = 2|x| log2(e) >= 1.8. The error of E reaches the result scaled by
+//            2/(E + 1)^2 * E, small enough that the whole function stays within 2 ulp. Large
+//            |x| needs no check: E overflows to +inf, its reciprocal is 0 and the result 1.
 //   exp is the device library's expf (branch free, 1 ulp), or csmi_expf_np where the
 //            argument cannot overflow (the same steps without the overflow check).
 //
@@ -26,9 +25,9 @@
 
 __device__ __forceinline__ float csmi_tanhf_p(float x) {
 #pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
-  const float ax = fabsf(x);
-  const float a = !(ax > 9.5f) ? ax : 9.5f;  // keeps NaN; tanh rounds to 1 beyond 9.01
-  // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) in [1.8, 27.4]
+  const float a = fabsf(x);
+  // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) >= 1.8; for t >= 128, 2^t is +inf and
+  // the reciprocal 0, which gives the 1 that tanh rounds to beyond 9.01
   const float e = __builtin_amdgcn_exp2f(a * 2.88539004f);
   const float big = fmaf(__builtin_amdgcn_rcpf(e + 1.0f), -2.0f, 1.0f);
   // |x| < 0.625: a + a^3 P(a^2)
@@ -43,16 +42,17 @@ __device__ __forceinline__ float csmi_tanhf_p(float x) {
 }
 
 // exp for an argument that cannot exceed 88 (the generated code uses it where the argument
-// is the negation of a square or of an absolute value): 2^n 2^f from the extended-precision
-// argument x log2(e) = ph + pl (log2 e as a float pair), n = rint(ph), the hardware 2^f on
-// |f| <= 1/2 and an exact scaling that produces subnormal results; below -104 the result
-// rounds to +0 (as does -inf, for which ph - n would be NaN). No overflow check is needed.
+// is the negation of a square or of an absolute value): 2^n 2^f with n = rint(x log2 e), the
+// reduced argument f = x log2(e) - n from log2 e as a float pair L1 + L2 (FMA keeps x L1 - n
+// exact before its one rounding), the hardware 2^f on |f| <= 1/2 and an exact scaling that
+// produces subnormal results; below -104 the result rounds to +0 (as does -inf, for which f
+// would be NaN). No overflow check is needed.
 __device__ __forceinline__ float csmi_expf_np(float x) {
 #pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
-  const float ph = x * 1.44269502f;
-  const float pl = fmaf(x, 1.92596299e-08f, fmaf(x, 1.44269502f, -ph));
-  const float n = rintf(ph);
-  const float r = ldexpf(__builtin_amdgcn_exp2f((ph - n) + pl), static_cast<int>(n));
+  const float n = rintf(x * 1.44269502f);
+  // f = x log2(e) - n, |f| <= 1/2 + 2^-17: one rounding for x L1 - n, one for the x L2 term
+  const float f = fmaf(x, 1.92596299e-08f, fmaf(x, 1.44269502f, -n));
+  const float r = ldexpf(__builtin_amdgcn_exp2f(f), static_cast<int>(n));
   return x < -104.0f ? 0.0f : r;  // NaN fails the compare and r is NaN
 }
 

@@ -516,6 +516,9 @@ cs_status cs_stream_destroy(cs_stream *stream) {
     {
       std::lock_guard<std::recursive_mutex> l(stream->device->mu);
       stream->device->streams.erase(stream);
+      // Pending frees hold events recorded on this stream; they are waited for and destroyed
+      // while the stream still exists, so that no event outlives the stream it was recorded on.
+      stream->device->arena->settle();
     }
     CS_HIP(hipStreamDestroy(stream->stream));
     if (stream->scratch) CS_HIP(hipFree(stream->scratch));

@@ -76,3 +76,26 @@ def test_parameter_order_is_first_visit():
     assert [p.id for p in leaves] == ["b", "a", "c"]
     slots = [n.param for n in nodes if n.op == 2]
     assert slots == [0, 1, 2]
+
+
+def test_split_fold_recognised(offline_tensors):
+    """split(d) folded with + or max over one materialised tensor is an axis reduction;
+    folds out of order, with mixed ops, over views of views or over several tensors are not."""
+    import functools
+    from compute.tensors import _split_fold
+    T = offline_tensors
+    t = T.Tensor(np.zeros((3, 5, 4), np.float32))
+    for d in range(3):
+        for op, f in (("+", lambda a, b: a + b), ("max", T.Tensor.max)):
+            r = functools.reduce(f, list(t.split(d)))
+            m = _split_fold(r.closure, r.shape)
+            assert m is not None and m[1] == d and m[2] == op and m[0].id is t
+    s = list(t.split(1))
+    assert _split_fold((s[1] + s[0] + s[2] + s[3] + s[4]).closure, (3, 4)) is None  # order
+    assert _split_fold(T.Tensor.max(s[0] + s[1] + s[2] + s[3], s[4]).closure, (3, 4)) is None  # mixed
+    assert _split_fold((s[0] + s[1] + s[2] + s[3]).closure, (3, 4)) is None  # a slice missing
+    v = list(t.transpose().split(0))  # slices of a view: composed matrices
+    assert _split_fold(functools.reduce(lambda a, b: a + b, v).closure, (5, 3)) is None
+    u = T.Tensor(np.zeros((3, 5, 4), np.float32))
+    w = list(u.split(1))
+    assert _split_fold((s[0] + s[1] + w[2] + s[3] + s[4]).closure, (3, 4)) is None  # two tensors

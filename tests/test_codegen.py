@@ -46,6 +46,9 @@ def test_c2_views_fold_into_one_kernel(offline_tensors):
     assert "p0 + (g2_i0 * 65536 + g2_i1 * 256 + g2_i2)" in src  # permute(1,0,2)
     assert "(int)(c2[u] + -1)" in src  # translate(0,0,1)
     assert "p1[g5_i0]" in src  # bias aligned with dimension 0, one scalar load
+    # X and its translate share the aligned chunk at the vector's own position: two 16-byte
+    # loads of X per vector of four
+    assert src.count("l = *(const csmi_f4*)") == 2 and "t8[u][1] = g2_w0[u][0];" in src
     assert "csmi_expf_np(" in src  # exp(-(y*y)) cannot overflow
     assert "csmi_tanhf_p(" in src
     compiles(src)

@@ -124,3 +124,34 @@ def test_combine_is_deterministic(gpu_tensors, kind):
         assert abs(got64[0] - exact) <= 1e-12 * np.abs(a).astype(np.float64).sum()
     for h in (b, outs, outs64, small, small_out):
         h.release()
+
+
+@pytest.mark.parametrize("shape,d,op", [((64, 300), 1, "+"), ((64, 300), 1, "max"), ((5, 7, 33), 2, "+"),
+                                        ((40, 50), 0, "+"), ((3, 20, 17), 1, "+"), ((40, 50), 0, "max")])
+def test_split_fold_is_an_axis_reduction(shape, d, op):
+    """t.split(d).reduce(+ or max), the reference's way to write axis reductions: recognised
+    and run as one reduction (sums within 4*eps32*sum|x| of FP64, maxima exact); the max over
+    a non-innermost dimension stays a fused kernel, bit-exact with the fold."""
+    import functools
+    from compute import Tensors
+    with Tensors(0) as T:
+        n = int(np.prod(shape))
+        x = inputs.heavy_tails(inputs.normal(n, 91), 92).reshape(shape)
+        t = T.Tensor(x).doCache()
+        slices = list(t.split(d))
+        fold = functools.reduce((lambda a, b: a + b) if op == "+" else T.Tensor.max, slices)
+        before = T.launches
+        got = fold.flatArray().reshape(tuple(s for i, s in enumerate(shape) if i != d))
+        x64 = x.astype(np.float64)
+        if op == "+":
+            assert T.launches - before == 1
+            exact = x64.sum(axis=d)
+            bound = 4 * np.finfo(np.float32).eps * np.abs(x64).sum(axis=d)
+            assert (np.abs(got - exact) <= bound).all()
+        else:
+            want = x.max(axis=d)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+            if d != len(shape) - 1:
+                ref = fp64.evaluate(fold.closure, fold.shape, [x], "fp32").reshape(got.shape)
+                assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
+        t.release()

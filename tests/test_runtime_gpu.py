@@ -187,3 +187,23 @@ def test_release_during_capture(gpu_tensors):
     capi.check(lib.cs_graph_release(g))
     for b in big + [out]:
         b.release()
+
+
+def test_release_then_stream_destroyed():
+    """Buffers released while a second context's stream exists, that stream then destroyed:
+    later allocations that settle the pending releases still succeed (no release event is
+    waited for after the stream it was recorded on is gone)."""
+    from compute import Tensors
+    keep = Tensors(0)
+    try:
+        for round_ in range(6):
+            with Tensors(0) as T:
+                x = T.Tensor(np.full((256, 1024), float(round_), np.float32))
+                y = (x + x).doCache()
+                assert float(y.flatArray()[0]) == 2.0 * round_
+                y.release()
+            bufs = [alloc(keep, 8 << 20) for _ in range(3)]  # waits on the releases above
+            for b in bufs:
+                b.release()
+    finally:
+        keep.close()
