@@ -25,8 +25,9 @@ def test_small_known_answer(gpu_tensors, variant):
 
 @pytest.mark.parametrize("variant", sorted(VARIANTS))
 @pytest.mark.parametrize("mnk", [(1, 1, 1), (33, 65, 17), (128, 256, 512), (300, 200, 1000), (256, 512, 384),
-                                 (513, 257, 129)])
+                                 (513, 257, 129), (300, 200, 1024), (1, 65, 64)])
 def test_gemm_shapes(gpu_tensors, variant, mnk):
+    """Padded tiles, K tails, and row counts that end inside a tile."""
     T = gpu_tensors
     M, N, K = mnk
     a = inputs.uniform(M * K, 4001).reshape(M, K)
