@@ -149,3 +149,56 @@ def test_edges_and_odd_shapes(gpu_tensors):
         assert np.array_equal(e.flatArray().view(np.uint32), want.view(np.uint32)), shape
     empty = T.Tensor(np.zeros((0, 4), np.float32))
     assert (empty + empty).flatArray().size == 0
+
+
+SPECIAL = np.array([np.nan, np.inf, -np.inf, 0.0, -0.0, 1e-45, -1e-45, 1e-40, 1.17549435e-38, 3.4e38, -3.4e38,
+                    1.0, -1.0, 0.5, 88.7, 89.0, -87.0, -104.0, -110.0, 9.5, 20.0, 1e-7, 2.0, 7.0, -3.0],
+                   np.float32)
+
+
+def _same(got, want):
+    """Bit-exact, with every NaN equal to every NaN (payloads are not specified)."""
+    g, w = got.ravel(), want.ravel()
+    nan = np.isnan(w)
+    return np.array_equal(np.isnan(g), nan) and np.array_equal(g[~nan].view(np.uint32), w[~nan].view(np.uint32))
+
+
+@pytest.mark.parametrize("op", ["+", "-", "*", "/", "%", "min", "max"])
+def test_special_values_binary(gpu_tensors, op):
+    """Every pair of special values (NaN, infinities, signed zeros, subnormals, the extremes)
+    through a fused kernel with a broadcast view on each side: IEEE FP32 results bit-exact
+    (fmodf for %, fminf/fmaxf for min/max, as OpenCL C defines them), so no contraction,
+    flush or fast-math rewrite is present."""
+    T = gpu_tensors
+    n = SPECIAL.size
+    a = T.Tensor(SPECIAL.reshape(n, 1)).broadcast((n, n))
+    b = T.Tensor(SPECIAL.reshape(1, n)).broadcast((n, n))
+    f = {"+": lambda x, y: x + y, "-": lambda x, y: x - y, "*": lambda x, y: x * y, "/": lambda x, y: x / y,
+         "%": lambda x, y: x % y, "min": T.Tensor.min, "max": T.Tensor.max}[op]
+    e = f(a, b)
+    want = fp64.evaluate(e.closure, e.shape, [SPECIAL.reshape(n, 1), SPECIAL.reshape(1, n)], "fp32")
+    got = e.flatArray().reshape(n, n)
+    if op in ("min", "max"):
+        # fmin/fmax of +0 and -0 may return either zero
+        zeros = (want == 0) & (got == 0)
+        got, want = np.where(zeros, 0.0, got).astype(np.float32), np.where(zeros, 0.0, want).astype(np.float32)
+    assert _same(got, want), op
+
+
+@pytest.mark.parametrize("op", ["neg", "abs", "sqrt", "exp", "tanh", "log"])
+def test_special_values_unary(gpu_tensors, op):
+    """Unary ops on the special values: neg, abs and sqrt bit-exact; exp, tanh and log exact
+    where the result is special (NaN, infinities, zero, +-1) and within 2 ulp elsewhere."""
+    T = gpu_tensors
+    x = T.Tensor(SPECIAL)
+    e = {"neg": lambda t: -t, "abs": T.Tensor.abs, "sqrt": T.Tensor.sqrt, "exp": T.Tensor.exp,
+         "tanh": T.Tensor.tanh, "log": T.Tensor.log}[op](x)
+    want = fp64.evaluate(e.closure, e.shape, [SPECIAL], "fp32")
+    got = e.flatArray()
+    if op in ("neg", "abs", "sqrt"):
+        assert _same(got, want), op
+        return
+    special = np.isnan(want) | np.isinf(want) | (want == 0) | (np.abs(want) == 1)
+    assert _same(got[special], want[special]), (op, SPECIAL[special], got[special], want[special])
+    d = fp64.ulp_distance(got[~special], want[~special])
+    assert d.max() <= 2, (op, SPECIAL[~special][d > 2])
