@@ -150,6 +150,7 @@ CP = ctypes.c_char_p
 _PROTOS = {
     "cs_last_error": (SZ, [ctypes.c_char_p, SZ]),
     "cs_version": (CP, []),
+    "cs_status_string": (CP, [I]),
     "cs_device_count": (I, [ctypes.POINTER(I)]),
     "cs_device_open": (I, [I, PP]),
     "cs_device_close": (I, [P]),
@@ -183,6 +184,7 @@ _PROTOS = {
     "cs_program_release": (I, [P]),
     "cs_program_get_kernel": (I, [P, CP, PP]),
     "cs_kernel_release": (I, [P]),
+    "cs_kernel_max_block": (I, [P, ctypes.POINTER(ctypes.c_int)]),
     "cs_launch": (I, [P, P, ctypes.POINTER(U32), ctypes.POINTER(U32), U32, PP, I]),
     "cs_free": (None, [P]),
     "cs_graph_begin_capture": (I, [P]),

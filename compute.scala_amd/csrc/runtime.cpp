@@ -402,6 +402,28 @@ size_t cs_last_error(char *buf, size_t buf_size) {
   return m.size();
 }
 
+const char *cs_status_string(cs_status status) {
+  switch (status) {
+    case CS_SUCCESS: return "CS_SUCCESS";
+    case CS_DEVICE_NOT_FOUND: return "CS_DEVICE_NOT_FOUND";
+    case CS_MEM_OBJECT_ALLOCATION_FAILURE: return "CS_MEM_OBJECT_ALLOCATION_FAILURE";
+    case CS_OUT_OF_RESOURCES: return "CS_OUT_OF_RESOURCES";
+    case CS_OUT_OF_HOST_MEMORY: return "CS_OUT_OF_HOST_MEMORY";
+    case CS_BUILD_PROGRAM_FAILURE: return "CS_BUILD_PROGRAM_FAILURE";
+    case CS_INVALID_VALUE: return "CS_INVALID_VALUE";
+    case CS_INVALID_DEVICE: return "CS_INVALID_DEVICE";
+    case CS_INVALID_MEM_OBJECT: return "CS_INVALID_MEM_OBJECT";
+    case CS_INVALID_PROGRAM: return "CS_INVALID_PROGRAM";
+    case CS_INVALID_KERNEL_NAME: return "CS_INVALID_KERNEL_NAME";
+    case CS_INVALID_KERNEL_ARGS: return "CS_INVALID_KERNEL_ARGS";
+    case CS_INVALID_WORK_GROUP_SIZE: return "CS_INVALID_WORK_GROUP_SIZE";
+    case CS_INVALID_EVENT: return "CS_INVALID_EVENT";
+    case CS_INVALID_OPERATION: return "CS_INVALID_OPERATION";
+    case CS_INVALID_BUFFER_SIZE: return "CS_INVALID_BUFFER_SIZE";
+    default: return "CS_UNKNOWN_STATUS";
+  }
+}
+
 const char *cs_version(void) { return "csmi 0.1 gfx950"; }
 
 void cs_free(void *p) { std::free(p); }
@@ -843,6 +865,17 @@ cs_status cs_program_release(cs_program *program) {
       (void)hipModuleUnload(program->module);
       delete program;
     }
+  });
+}
+
+cs_status cs_kernel_max_block(cs_kernel *kernel, int *max_block) {
+  return guard([&] {
+    if (!kernel) fail(CS_INVALID_VALUE, "kernel is null");
+    if (!max_block) fail(CS_INVALID_VALUE, "max_block is null");
+    set_device(kernel->program->device);
+    int v = 0;
+    CS_HIP(hipFuncGetAttribute(&v, HIP_FUNC_ATTRIBUTE_MAX_THREADS_PER_BLOCK, kernel->function));
+    *max_block = v;
   });
 }
 

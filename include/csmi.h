@@ -57,6 +57,11 @@ typedef int cs_status;
  * and returns its full length. Replaces the exception messages of OpenCL.scala:143-312. */
 size_t cs_last_error(char *buf, size_t buf_size);
 
+/* The name of a status code ("CS_BUILD_PROGRAM_FAILURE", ...; "CS_UNKNOWN_STATUS" for any
+ * other value), a static string. The JNI shim maps codes to exception classes by value
+ * (OpenCL.scala:143-312); this name is for messages. */
+const char *cs_status_string(cs_status status);
+
 /* Library and build identification ("csmi <version> gfx950"). */
 const char *cs_version(void);
 
@@ -152,6 +157,10 @@ cs_status cs_program_build(cs_device *device, const char *source, const char *op
 cs_status cs_program_release(cs_program *program);
 cs_status cs_program_get_kernel(cs_program *program, const char *name, cs_kernel **kernel);
 cs_status cs_kernel_release(cs_kernel *kernel);
+/* The largest workgroup size `kernel` can be launched with (its launch bounds and register
+ * use), the `kernel.workGroupSize(deviceId)` query of Tensors.scala:700, 738
+ * (CL_KERNEL_WORK_GROUP_SIZE, OpenCL.scala:739-753). */
+cs_status cs_kernel_max_block(cs_kernel *kernel, int *max_block);
 /* Launches `kernel` with hipModuleLaunchKernel semantics: args[i] points at the value of
  * argument i. grid is in workgroups. */
 cs_status cs_launch(cs_stream *stream, cs_kernel *kernel, const uint32_t grid[3],

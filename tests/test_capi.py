@@ -37,6 +37,15 @@ def test_version_and_errors_without_device():
         assert "ordinal" in capi.last_error()
 
 
+def test_status_names():
+    assert capi.lib.cs_status_string(0) == b"CS_SUCCESS"
+    assert capi.lib.cs_status_string(-11) == b"CS_BUILD_PROGRAM_FAILURE"
+    assert capi.lib.cs_status_string(-54) == b"CS_INVALID_WORK_GROUP_SIZE"
+    assert capi.lib.cs_status_string(-2) == b"CS_UNKNOWN_STATUS"
+    for status, cls in capi._BY_STATUS.items():  # every mapped code has its own name
+        assert capi.lib.cs_status_string(status) != b"CS_UNKNOWN_STATUS", cls
+
+
 def test_null_arguments_are_rejected():
     assert capi.lib.cs_stream_synchronize(None) == -30
     assert capi.lib.cs_buffer_release(None) == -38

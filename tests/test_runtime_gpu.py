@@ -72,6 +72,9 @@ def test_program_build_and_launch(gpu_tensors):
     capi.check(lib.cs_program_get_kernel(prog, b"axpy", ctypes.byref(k)))
     with pytest.raises(capi.InvalidKernelName):
         capi.check(lib.cs_program_get_kernel(prog, b"nope", ctypes.byref(ctypes.c_void_p())))
+    mb = ctypes.c_int(0)
+    capi.check(lib.cs_kernel_max_block(k, ctypes.byref(mb)))
+    assert 256 <= mb.value <= 1024  # kernel.workGroupSize(deviceId)
     n = 1000
     x, y = np.arange(n, dtype=np.float32), np.ones(n, np.float32)
     bx, by = upload(T, x), upload(T, y)
