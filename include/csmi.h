@@ -159,7 +159,7 @@ cs_status cs_program_get_kernel(cs_program *program, const char *name, cs_kernel
 cs_status cs_kernel_release(cs_kernel *kernel);
 /* The largest workgroup size `kernel` can be launched with (its launch bounds and register
  * use), the `kernel.workGroupSize(deviceId)` query of Tensors.scala:700, 738
- * (CL_KERNEL_WORK_GROUP_SIZE, OpenCL.scala:739-753). */
+ * (CL_KERNEL_WORK_GROUP_SIZE, OpenCL.scala:739-749). */
 cs_status cs_kernel_max_block(cs_kernel *kernel, int *max_block);
 /* Launches `kernel` with hipModuleLaunchKernel semantics: args[i] points at the value of
  * argument i. grid is in workgroups. */
