@@ -199,6 +199,7 @@ _PROTOS = {
     "cs_fused_signature": (I, [P, ctypes.POINTER(I), ctypes.POINTER(I64)]),
     "cs_fused_key": (I, [P, ctypes.POINTER(U64)]),
     "cs_fused_launch": (I, [P, P, PP, I, P]),
+    "cs_fused_launch_timed": (I, [P, P, PP, I, P, PP, PP]),
     "cs_fused_lookup": (I, [P, ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I,
                             ctypes.POINTER(I32), ctypes.POINTER(CsFusedOptions), ctypes.POINTER(I)]),
     "cs_fused_cache_clear": (I, [P]),

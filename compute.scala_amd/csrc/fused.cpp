@@ -14,6 +14,8 @@
 #include <cstdio>
 #include <sstream>
 
+#include <hip/hip_ext.h>
+
 #include "internal.h"
 #include "precise_math.inc"  // kPreciseMathSource, generated from precise_math.h
 
@@ -1250,6 +1252,48 @@ cs_status cs_fused_launch(cs_stream *stream, cs_fused *fused, const void *const 
     set_device(stream->device);
     CS_HIP(hipModuleLaunchKernel(fused->function, fused->grid, 1, 1, fused->block, 1, 1, 0,
                                  stream->stream, args.data(), nullptr));
+  });
+}
+
+cs_status cs_fused_launch_timed(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
+                                void *out, cs_event **start, cs_event **stop) {
+  return guard([&] {
+    if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (!start || !stop) fail(CS_INVALID_VALUE, "null event pointer");
+    if (nleaves != fused->nleaves)
+      fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
+                                       std::to_string(nleaves));
+    if (fused->grid == 0) fail(CS_INVALID_OPERATION, "an empty output launches no kernel to time");
+    if (!out) fail(CS_INVALID_KERNEL_ARGS, "output pointer is null");
+    std::vector<const void *> vals(static_cast<size_t>(nleaves) + 1);
+    std::vector<void *> args(static_cast<size_t>(nleaves) + 1);
+    for (int i = 0; i < nleaves; ++i) {
+      if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
+      vals[i] = leaves[i];
+      args[i] = &vals[i];
+    }
+    vals[nleaves] = out;
+    args[nleaves] = &vals[nleaves];
+    set_device(stream->device);
+    hipEvent_t ea = nullptr, eb = nullptr;
+    hipError_t r = hipEventCreate(&ea);
+    if (r == hipSuccess) r = hipEventCreate(&eb);
+    // hipExtModuleLaunchKernel takes the grid in work items; the events carry the
+    // dispatch's own start and end
+    if (r == hipSuccess)
+      r = hipExtModuleLaunchKernel(fused->function, fused->grid * fused->block, 1, 1, fused->block, 1, 1, 0,
+                                   stream->stream, args.data(), nullptr, ea, eb, 0);
+    if (r != hipSuccess) {
+      if (ea) (void)hipEventDestroy(ea);
+      if (eb) (void)hipEventDestroy(eb);
+      CS_HIP(r);
+    }
+    auto *a = new cs_event(), *b = new cs_event();
+    a->ordinal = b->ordinal = stream->device->ordinal;
+    a->event = ea;
+    b->event = eb;
+    *start = a;
+    *stop = b;
   });
 }
 

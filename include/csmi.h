@@ -265,6 +265,13 @@ typedef struct cs_fused_desc {
   cs_fused_options options;
 } cs_fused_desc;
 cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out);
+/* cs_fused_launch that also returns two new events (refcount 1) stamped with the kernel
+ * dispatch's own start and end, as OpenCL's profiling info gives for a command
+ * (CL_PROFILING_COMMAND_START/END; the reference maps CL_PROFILING_INFO_NOT_AVAILABLE at
+ * OpenCL.scala:259): cs_event_elapsed_ms(*start, *stop) is the kernel's duration, without
+ * the launch gap before it. Not for use under graph capture. */
+cs_status cs_fused_launch_timed(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
+                                void *out, cs_event **start, cs_event **stop);
 /* cs_fused_launch with the workgroup size chosen at launch (0 = the compiled default):
  * the generated code reads the workgroup size at run time, so one compile serves every
  * size the autotuner tries. */

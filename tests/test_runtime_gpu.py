@@ -210,3 +210,33 @@ def test_release_then_stream_destroyed():
                 b.release()
     finally:
         keep.close()
+
+
+def test_timed_launch_matches_and_times(gpu_tensors):
+    """cs_fused_launch_timed computes what cs_fused_launch computes and returns events
+    stamped by the dispatch: a positive duration below the host-side wall time."""
+    import time
+    T = gpu_tensors
+    x = np.arange(1 << 20, dtype=np.float32)
+    t = T.Tensor(x).doCache()
+    e = t * t + t
+    f, leaves, _ = T.compile(e.closure, e.shape)
+    out1, out2 = alloc(T, x.size), alloc(T, x.size)
+    ptrs = (ctypes.c_void_p * 1)(t.buffer.ptr)
+    capi.check(lib.cs_fused_launch(T.stream, f, ptrs, 1, out1.ptr))
+    e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+    T.synchronize()
+    w0 = time.perf_counter()
+    capi.check(lib.cs_fused_launch_timed(T.stream, f, ptrs, 1, out2.ptr, ctypes.byref(e0), ctypes.byref(e1)))
+    T.synchronize()
+    wall_ms = (time.perf_counter() - w0) * 1e3
+    ms = ctypes.c_float()
+    capi.check(lib.cs_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+    assert 0.0 < ms.value < wall_ms
+    assert np.array_equal(download(T, out1, x.size).view(np.uint32), download(T, out2, x.size).view(np.uint32))
+    for ev in (e0, e1):
+        capi.check(lib.cs_event_release(ev))
+    lib.cs_fused_release(f)
+    out1.release()
+    out2.release()
+    t.release()
