@@ -443,8 +443,15 @@ class Emitter {
     s << "  }\n";
   }
 
+  // Non-temporal stores also write through at device scope (`sc1 nt`, a vector store): the
+  // output is not read again, and no dirty line is left in L2 for the release at the end of
+  // the kernel, which otherwise writes it back between this launch and the next.
   std::string store(const std::string &addr, const std::string &val, const std::string &type) {
-    if (o_.nt) return "__builtin_nontemporal_store(" + val + ", (" + gq_ + type + "*)(" + addr + "));";
+    if (o_.nt) {
+      const std::string op = type == "csmi_v4" ? "global_store_dwordx4" : "global_store_dword";
+      return "asm volatile(\"" + op + " %0, %1, off sc1 nt\" :: \"v\"((" + gq_ + type + "*)(" + addr +
+             ")), \"v\"(" + val + ") : \"memory\");";
+    }
     return "*(" + gq_ + type + "*)(" + addr + ") = " + val + ";";
   }
 

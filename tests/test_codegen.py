@@ -174,3 +174,19 @@ def test_batch_source_offline(offline_tensors):
     compiles(text)
     with pytest.raises(capi.InvalidWorkGroupSize):
         capi.check(capi.lib.cs_batch_generate_source(descs, len(exprs), 96, ctypes.byref(src), None))
+
+
+@pytest.mark.parametrize("v", [1, 2, 4, 8])
+def test_nontemporal_stores_write_through(v):
+    """With non-temporal stores the outputs leave through vector stores that write through at
+    device scope (`sc1 nt`): one 16-byte store per 4 floats of a run, single floats otherwise."""
+    from compute import Tensors
+    T = Tensors(offline=True, vector_width=v)
+    T.options.nontemporal_store = 1
+    x = T.Tensor(np.zeros((3, 7, 8), np.float32))
+    j = T.Tensor.join([x + x, x * x])  # two outputs: runs of 2V floats
+    src = T.kernel_source(trees.Concatenate([p.closure for p in j._parts]), (3, 7, 8))
+    run = 2 * v
+    assert src.count("global_store_dwordx4 %0, %1, off sc1 nt") == (run // 4 if run % 4 == 0 else 0)
+    assert src.count("global_store_dword %0, %1, off sc1 nt") == (0 if run % 4 == 0 else run)
+    compiles(src)
