@@ -290,7 +290,7 @@ def bench_gemm(T, capi, variants, reps=10):
     return out
 
 
-def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001):
+def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=0):
     """512 uneven expressions: per-expression launches vs one persistent batch (both
     graph-captured), with configurations from the cost model when its parameters exist.
     With `sweep`, first runs the exhaustive autotuner over every configuration, trains the
@@ -311,7 +311,8 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001):
     t0 = time.time()
     with concurrent.futures.ThreadPoolExecutor(len(variants)) as ex:
         futs = {vu: ex.submit(bm.compile_many, T, exprs, [{"vector_width": vu[0], "items_per_thread": vu[1],
-                                                           "block_size": 256}] * len(exprs)) for vu in variants}
+                                                           "block_size": 256, "nontemporal_store": nontemporal}] *
+                              len(exprs)) for vu in variants}
         fused = {vu: f.result() for vu, f in futs.items()}
     res["compile_s_all_variants"] = time.time() - t0
     log("c5: compiled %d variants x %d expressions in %.1f s" % (len(variants), len(exprs), res["compile_s_all_variants"]))
@@ -401,7 +402,8 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001):
                                      for x, y in zip(each_out, batch_out))
     gb.release()
     res["batch_info"] = b.info()
-    res["batch_config"] = {"vector_width": batch_vu[0], "items_per_thread": batch_vu[1], "block_size": 512}
+    res["batch_config"] = {"vector_width": batch_vu[0], "items_per_thread": batch_vu[1], "block_size": 512,
+                           "nontemporal_store": nontemporal}
     b.release()
     res["per_expression_default_us"] = us_def
     res["per_expression_us"] = us_each
@@ -456,6 +458,9 @@ def main():
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
     ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
+    ap.add_argument("--c5-nontemporal", type=int, default=0,
+                    help="c5: compile every expression with write-through non-temporal stores (1; measured "
+                         "slower on these small, uneven outputs: batch 1441 us against 957 us) or plain (0)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")
@@ -499,7 +504,8 @@ def main():
         if "bf16x3" in g and "fp32" in g:
             g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
     if "c5" in only:
-        also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir)
+        also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir,
+                                           nontemporal=args.c5_nontemporal)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_c2()

@@ -41,7 +41,8 @@ def _descs(exprs, configs, math_mode):
         last = e.shape[-1] if e.shape else 1
         while v > 1 and last % v:
             v //= 2
-        d.options = capi.CsFusedOptions(math_mode, v, cfg["items_per_thread"], cfg.get("block_size", 256), 0)
+        d.options = capi.CsFusedOptions(math_mode, v, cfg["items_per_thread"], cfg.get("block_size", 256),
+                                        cfg.get("nontemporal_store", 0))
     return descs, keep
 
 
