@@ -93,7 +93,7 @@ __device__ __forceinline__ void block_reduce(Acc &a, float &m) {
 }
 
 // One workgroup of kFinish threads adds partials[0..n) in FP64 in a fixed order: thread t
-// adds pairs 2t, 2t + 2 kFinish, ... (16-byte loads, four in flight), then a fixed xor-shuffle
+// adds pairs 2t, 2t + 2 kFinish, ... (16-byte loads, eight in flight), then a fixed xor-shuffle
 // tree inside each wave and the 16 waves in order.
 constexpr int kFinish = 1024;
 typedef double d2 __attribute__((ext_vector_type(2)));
@@ -104,12 +104,13 @@ __global__ __launch_bounds__(kFinish) void finish_sum_kernel(const double *__res
   const int64_t pairs = n / 2;
   const d2 *p2 = reinterpret_cast<const d2 *>(partials);  // partials are 16-byte aligned
   int64_t i = threadIdx.x;
-  for (; i + 3 * kFinish < pairs; i += 4 * kFinish) {
-    d2 v[4];
+  // eight 16-byte loads in flight per thread: one memory round trip for up to 16384 partials
+  for (; i + 7 * kFinish < pairs; i += 8 * kFinish) {
+    d2 v[8];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) v[k] = p2[i + k * kFinish];
+    for (int k = 0; k < 8; ++k) v[k] = p2[i + k * kFinish];
 #pragma unroll
-    for (int k = 0; k < 4; ++k) acc += v[k].x + v[k].y;
+    for (int k = 0; k < 8; ++k) acc += v[k].x + v[k].y;
   }
   for (; i < pairs; i += kFinish) {
     const d2 v = p2[i];
