@@ -597,6 +597,12 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
 
 int64_t round_up(int64_t v, int64_t a) { return (v + a - 1) / a * a; }
 
+__global__ __launch_bounds__(256) void zero_matrix_kernel(float *__restrict__ C, int M, int N, int ldc) {
+  const int64_t i = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= int64_t(M) * N) return;
+  C[(i / N) * ldc + i % N] = 0.0f;
+}
+
 template <int KIND>
 void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, const float *B, int ldb, float *C,
              int ldc) {
@@ -660,16 +666,23 @@ extern "C" cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K
     if (variant < CS_GEMM_FP32 || variant > CS_GEMM_FP8)
       fail(CS_INVALID_VALUE, "unknown GEMM variant " + std::to_string(variant));
     if (M == 0 || N == 0) return;
-    if (!A || !B || !C) fail(CS_INVALID_VALUE, "null matrix pointer");
+    if (!C) fail(CS_INVALID_VALUE, "null matrix pointer");
+    set_device(stream->device);
+    if (K == 0) {  // empty sums: C = 0, and A and B are not read
+      const int64_t n = int64_t(M) * N;
+      hipLaunchKernelGGL(zero_matrix_kernel, dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream->stream, C, M, N,
+                         ldc);
+      CS_HIP(hipGetLastError());
+      return;
+    }
+    if (!A || !B) fail(CS_INVALID_VALUE, "null matrix pointer");
     if (int64_t(M) * K >= (int64_t(1) << 31) || int64_t(N) * K >= (int64_t(1) << 31))
       fail(CS_INVALID_VALUE, "operands are limited to 2^31 elements");
-    set_device(stream->device);
-    const int Ke = K == 0 ? 1 : K;  // K = 0: a zero-padded stage gives C = 0
     switch (variant) {
-      case CS_GEMM_FP32: run_lds<kF32>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_BF16: run_lds<kBF16>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_BF16X3: run_lds<kBF16X3>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
-      case CS_GEMM_FP8: run_lds<kFP8>(stream, M, N, Ke, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_FP32: run_lds<kF32>(stream, M, N, K, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_BF16: run_lds<kBF16>(stream, M, N, K, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_BF16X3: run_lds<kBF16X3>(stream, M, N, K, A, lda, B, ldb, C, ldc); break;
+      case CS_GEMM_FP8: run_lds<kFP8>(stream, M, N, K, A, lda, B, ldb, C, ldc); break;
     }
   });
 }

@@ -28,12 +28,13 @@ def xorshift(s):
     return t2 ^ (t2 << np.uint32(5))
 
 
-def _index(n):
-    return np.arange(n, dtype=np.uint64).astype(np.uint32)
+def _index(n, start=0):
+    return np.arange(start, start + n, dtype=np.uint64).astype(np.uint32)
 
 
-def uniform(n, seed, lo=-1.0, hi=1.0):
-    h = wang_hash(_index(n) ^ np.uint32(seed & 0xFFFFFFFF))
+def uniform(n, seed, lo=-1.0, hi=1.0, start=0):
+    """Elements start .. start + n - 1 of the sequence."""
+    h = wang_hash(_index(n, start) ^ np.uint32(seed & 0xFFFFFFFF))
     u = h.astype(np.float64) * (1.0 / 4294967296.0)
     return (lo + (hi - lo) * u).astype(np.float32)
 

@@ -202,3 +202,29 @@ def test_special_values_unary(gpu_tensors, op):
     assert _same(got[special], want[special]), (op, SPECIAL[special], got[special], want[special])
     d = fp64.ulp_distance(got[~special], want[~special])
     assert d.max() <= 2, (op, SPECIAL[~special][d > 2])
+
+
+def test_output_near_the_index_limit():
+    """An output of 2^31 - 2^20 elements (8 GiB), just under the 32-bit index limit that the
+    generated kernels share with the reference's (an int global id): y = x*2 + 1 over a
+    U(-1,1) leaf generated on the device, three 1 Mi-element windows (first, middle, last)
+    bit-exact against the oracle's sequence."""
+    import ctypes
+    from compute import Tensors, _capi as capi
+    from compute import tensors as tz
+    with Tensors(0) as T:
+        n = (1 << 31) - (1 << 20)
+        buf = T._alloc(n * 4)
+        capi.check(capi.lib.cs_fill_uniform(T.stream, buf.ptr, n, 77, -1.0, 1.0))
+        x = tz.CachedTensor(T, (n,), 0.0, buf)
+        y = (x * T.Tensor.fill(2.0, (n,)) + T.Tensor.fill(1.0, (n,))).doCache()
+        m = 1 << 20
+        for start in (0, n // 2, n - m):
+            got = np.empty(m, np.float32)
+            capi.check(capi.lib.cs_buffer_read(T.stream, y.buffer.handle, start * 4,
+                                               got.ctypes.data_as(ctypes.c_void_p), m * 4))
+            xs = inputs.uniform(m, 77, start=start)
+            want = xs * np.float32(2.0) + np.float32(1.0)
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), start
+        y.release()
+        x.release()

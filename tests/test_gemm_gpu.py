@@ -74,3 +74,15 @@ def test_fp8_representation(gpu_tensors):
     assert np.array_equal(c, (a.astype(np.float64) @ b.astype(np.float64)).astype(np.float32))
     for h in (ba, bb, bc):
         h.release()
+
+
+@pytest.mark.parametrize("variant", sorted(VARIANTS))
+def test_empty_inner_dimension(gpu_tensors, variant):
+    """K = 0: every output is an empty sum, C = 0, and A and B are not read (null here)."""
+    T = gpu_tensors
+    M, N = 70, 300
+    bc = upload(T, np.full(M * N, 7.0, np.float32))
+    capi.check(capi.lib.cs_gemm(T.stream, VARIANTS[variant], M, N, 0, None, 0, None, N, bc.ptr, N))
+    c = download(T, bc, M * N)
+    assert np.array_equal(c, np.zeros(M * N, np.float32))
+    bc.release()
