@@ -336,9 +336,16 @@ cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64
     if (rows < 0 || cols < 0) fail(CS_INVALID_VALUE, "negative shape");
     if (rows > 0x7fffffff) fail(CS_INVALID_VALUE, "more than 2^31-1 rows");
     if (rows > 0 && cols > 0 && !in) fail(CS_INVALID_VALUE, "input is null");
-    if (rows == 0) return;
     set_device(stream->device);
     bool want_total = total || total64;
+    if (rows == 0) {
+      // no rows: the total is an empty sum, written by the combine over zero partials
+      if (want_total)
+        hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, nullptr, int64_t(0), total,
+                           total64);
+      CS_HIP(hipGetLastError());
+      return;
+    }
     ReduceScratch s = scratch_for(stream, want_total ? static_cast<size_t>(rows) : 0);
     bool vec = cols % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
     auto k = vec ? reduce_rows_kernel<true> : reduce_rows_kernel<false>;
@@ -359,7 +366,7 @@ cs_status cs_reduce_cols(cs_stream *stream, const float *in, int64_t rows, int64
     if (!col_sum || (rows > 0 && !in)) fail(CS_INVALID_VALUE, "null pointer");
     set_device(stream->device);
     int64_t groups = (cols + kBlock - 1) / kBlock;
-    if (groups > 1024) fail(CS_INVALID_VALUE, "more than 262144 columns");
+    if (groups > 0x7fffffff) fail(CS_INVALID_VALUE, "too many columns");
     int cus = stream->device->props.multiProcessorCount;
     int64_t slices = (int64_t(cus) * 4 + groups - 1) / groups;
     int64_t max_slices = (rows + 63) / 64;

@@ -72,7 +72,7 @@ def test_rows_full_size(gpu_tensors):
         h.release()
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 1023), (129, 4099), (1000, 3)])
+@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 1023), (129, 4099), (1000, 3), (3, 300001)])
 def test_rows_and_cols_ragged(gpu_tensors, shape):
     T = gpu_tensors
     rows, cols = shape
@@ -155,3 +155,18 @@ def test_split_fold_is_an_axis_reduction(shape, d, op):
                 ref = fp64.evaluate(fold.closure, fold.shape, [x], "fp32").reshape(got.shape)
                 assert np.array_equal(got.view(np.uint32), ref.view(np.uint32))
         t.release()
+
+
+def test_rows_of_empty_matrices(gpu_tensors):
+    """No rows: the total is 0 (written, not left as it was); no columns: row sums 0."""
+    T = gpu_tensors
+    tot, tot64 = upload(T, np.array([5.0], np.float32)), alloc(T, 1, np.float64)
+    capi.check(capi.lib.cs_memcpy_htod(T.stream, ctypes.c_void_p(tot64.ptr),
+                                       np.array([5.0]).ctypes.data_as(ctypes.c_void_p), 8))
+    capi.check(capi.lib.cs_reduce_rows(T.stream, None, 0, 7, None, None, tot.ptr, tot64.ptr))
+    assert download(T, tot, 1)[0] == 0.0 and download(T, tot64, 1, np.float64)[0] == 0.0
+    rs = upload(T, np.full(3, 5.0, np.float32))
+    capi.check(capi.lib.cs_reduce_rows(T.stream, None, 3, 0, rs.ptr, None, tot.ptr, None))
+    assert np.array_equal(download(T, rs, 3), np.zeros(3, np.float32)) and download(T, tot, 1)[0] == 0.0
+    for h in (tot, tot64, rs):
+        h.release()
