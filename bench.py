@@ -215,12 +215,21 @@ def bench_c1(T, capi, steps, sets=16):
                                   for i in range(sets)])
     timer.measure(lambda: lib.cs_graph_launch(T.stream, g2), 2)
     ms_sum = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g2)), reps)
+    # reported separately: the sum fused into its producer, one kernel reading a, b, c once
+    # (12 MiB + 4 B) with no intermediate tensor, then the FP64 combine
+    fs, _ = T.compile_sum(e.closure, e.shape)
+    g3 = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_fused_sum(T.stream, fs, ptrs[i], 3, sums[i].ptr, None))
+                                  for i in range(sets)])
+    timer.measure(lambda: lib.cs_graph_launch(T.stream, g3), 2)
+    ms_fsum = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g3)), reps)
     L = reps * sets
     res = {"fma_gbs": 16 * n * L / (ms_fma * 1e-3) / 1e9, "fma_us": ms_fma * 1e3 / L, "fma_config": results[0][0],
-           "sum_gbs": (4 * n + 4) * L / (ms_sum * 1e-3) / 1e9, "sum_us": ms_sum * 1e3 / L}
-    for g in (g1, g2):
+           "sum_gbs": (4 * n + 4) * L / (ms_sum * 1e-3) / 1e9, "sum_us": ms_sum * 1e3 / L,
+           "fused_fma_sum_us": ms_fsum * 1e3 / L, "fused_fma_sum_gbs": (12 * n + 4) * L / (ms_fsum * 1e-3) / 1e9}
+    for g in (g1, g2, g3):
         lib.cs_graph_release(g)
     lib.cs_fused_release(f)
+    lib.cs_fused_release(fs)
     for t in (ta, tb, tc):
         t.release()
     for h in [x for s in a for x in s] + outs + sums:

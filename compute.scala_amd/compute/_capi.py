@@ -200,6 +200,9 @@ _PROTOS = {
     "cs_fused_key": (I, [P, ctypes.POINTER(U64)]),
     "cs_fused_launch": (I, [P, P, PP, I, P]),
     "cs_fused_launch_timed": (I, [P, P, PP, I, P, PP, PP]),
+    "cs_fused_compile_sum": (I, [P, ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I,
+                                 ctypes.POINTER(I32), ctypes.POINTER(CsFusedOptions), PP]),
+    "cs_fused_sum": (I, [P, P, PP, I, P, P]),
     "cs_fused_lookup": (I, [P, ctypes.POINTER(CsNode), I, ctypes.POINTER(I32), I, I,
                             ctypes.POINTER(I32), ctypes.POINTER(CsFusedOptions), ctypes.POINTER(I)]),
     "cs_fused_cache_clear": (I, [P]),

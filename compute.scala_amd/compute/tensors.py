@@ -494,8 +494,29 @@ class _ReduceTensor(NonInlineTensor):
         self._source = source
         self._kind = kind
 
+    def _fusable(self, memo):
+        t = self._tensors
+        s = self._source
+        return (self._kind == "sum" and t.fused_sums and isinstance(s, InlineTensor) and id(s) not in memo and
+                not isinstance(s.closure, trees.Concatenate))
+
     def _materialize(self, memo):
         t = self._tensors
+        if self._fusable(memo):
+            f, leaves = t.compile_sum(self._source.closure, self._source.shape)
+            try:
+                args = [leaf.id._buffer(memo) for leaf in leaves]
+                try:
+                    out = t._alloc(4)
+                    ptrs = (ctypes.c_void_p * max(len(args), 1))(*[a.ptr for a in args])
+                    check(lib.cs_fused_sum(t._stream, f, ptrs, len(args), out.ptr, None), "cs_fused_sum")
+                    t.launches += 1
+                    return out
+                finally:
+                    for a in args:
+                        a.release()
+            finally:
+                lib.cs_fused_release(f)
         src = self._source._buffer(memo)
         try:
             out = t._alloc(4)
@@ -741,11 +762,14 @@ class Tensors(object):
     Mirror of `trait Tensors` plus the plug-in wiring of `object gpu` (gpu.scala:15-27)."""
 
     def __init__(self, ordinal=0, math_mode=capi.MATH_PRECISE, vector_width=0, items_per_thread=0,
-                 block_size=0, offline=False, axis_reductions=True):
+                 block_size=0, offline=False, axis_reductions=True, fused_sums=True):
         self.options = capi.CsFusedOptions(math_mode, vector_width, items_per_thread, block_size, 0)
         # split(d).reduce(+ or max) folds run as one axis reduction (compensated FP32 sums
         # combined in FP64, exact maxima) instead of one kernel with a term per slice
         self.axis_reductions = axis_reductions
+        # `sum` of an inline tensor runs as one kernel that computes and adds its elements,
+        # without writing them (the reference materialises the tensor first)
+        self.fused_sums = fused_sums
         self.Tensor = TensorCompanion(self)
         self._lock = threading.RLock()
         self.launches = 0
@@ -831,6 +855,17 @@ class Tensors(object):
         check(lib.cs_fused_compile(self._device, nodes, len(nodes), outs, len(outputs), len(shape), shp,
                                    ctypes.byref(self.options), ctypes.byref(f)), "cs_fused_compile")
         return f.value, leaves, len(outputs)
+
+    def compile_sum(self, closure, shape):
+        """The fused-sum kernel of a single-output `closure` over `shape` (cs_fused_compile_sum);
+        returns the handle (caller releases) and the leaves in argument order."""
+        nodes, outputs, leaves = trees.serialize(closure)
+        outs = (ctypes.c_int32 * len(outputs))(*outputs)
+        shp = (ctypes.c_int32 * max(len(shape), 1))(*shape)
+        f = ctypes.c_void_p()
+        check(lib.cs_fused_compile_sum(self._device, nodes, len(nodes), outs, len(outputs), len(shape), shp,
+                                       ctypes.byref(self.options), ctypes.byref(f)), "cs_fused_compile_sum")
+        return f.value, leaves
 
     def kernel_source(self, closure, shape):
         if self.offline:

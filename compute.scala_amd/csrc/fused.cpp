@@ -245,6 +245,7 @@ std::string float_literal(float v) {
 struct Options {
   int math_mode = CS_MATH_PRECISE;
   int V = 0, U = 0, block = 0, nt = 0;
+  bool sum = false;  // the kernel adds its single output into FP64 block partials instead of storing it
 };
 
 Options resolve_options(const cs_fused_options *o, const Dag &dag) {
@@ -310,6 +311,7 @@ std::string canonical_key(const Dag &dag, const Options &o) {
   put(k, o.U);
   put(k, o.block);
   put(k, o.nt);
+  if (o.sum) put(k, int32_t(1));  // appended only in sum mode: element-wise keys are unchanged
   return k;
 }
 
@@ -363,6 +365,12 @@ class Emitter {
     p += "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     p += "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
     p += "#define CSMI_G __attribute__((address_space(1)))\n";
+    // s + c += x with the rounding error of the addition kept in c (TwoSum); the steps are
+    // exact only without contraction or reassociation, whatever the math mode
+    p += "__device__ __forceinline__ void csmi_acc(float &s, float &c, float x) {\n"
+         "#pragma clang fp contract(off) reassociate(off)\n"
+         "  const float t = s + x;\n  const float bp = t - s;\n"
+         "  c += (s - (t - bp)) + (x - bp);\n  s = t;\n}\n";
     return p;
   }
 
@@ -380,7 +388,10 @@ class Emitter {
 
   void params(std::ostringstream &s) {
     for (int i = 0; i < dag_.nleaves; ++i) s << "const " << gq_ << "float* __restrict__ p" << i << ", ";
-    s << gq_ << "float* __restrict__ out";
+    if (o_.sum)
+      s << "double* __restrict__ partials";
+    else
+      s << gq_ << "float* __restrict__ out";
   }
 
   // tile: the index of the block's slice of blockDim.x * U vectors.
@@ -419,6 +430,10 @@ class Emitter {
       int op = dag_.nodes[i].op;
       if (op != CS_OP_EXTRACT && !is_array(op) && op != CS_OP_LITERAL) emit_arith(static_cast<int>(i));
     }
+    if (o_.sum) {
+      emit_block_sum(s);
+      return;
+    }
     // phase 3: stores -- out[(e0 + v) * nout + j] is one contiguous run of V * nout floats
     const int run = V * nout;
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
@@ -441,6 +456,26 @@ class Emitter {
       }
     }
     s << "  }\n";
+  }
+
+  // Sum mode, in place of the stores: each thread adds its elements with a compensation term
+  // (csmi_acc), the (sum, compensation) pairs become FP64 and are added by a fixed xor-shuffle
+  // tree in each wave and in wave order across the block, and thread 0 writes the block's
+  // partial. The element values are the ones the stores would write.
+  void emit_block_sum(std::ostringstream &s) {
+    const int V = o_.V, U = o_.U;
+    s << "  float csmi_s = 0.0f, csmi_c = 0.0f;\n";
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n    if (!live[u]) continue;\n";
+    for (int v = 0; v < V; ++v) s << "    csmi_acc(csmi_s, csmi_c, " << value_of(dag_.outputs[0], v) << ");\n";
+    s << "  }\n";
+    s << "  double csmi_d = (double)csmi_s + (double)csmi_c;\n";
+    s << "#pragma unroll\n  for (int off = 32; off >= 1; off >>= 1) csmi_d += __shfl_xor(csmi_d, off, 64);\n";
+    s << "  __shared__ double csmi_lw[16];\n";
+    s << "  if ((threadIdx.x & 63u) == 0u) csmi_lw[threadIdx.x >> 6] = csmi_d;\n";
+    s << "  __syncthreads();\n";
+    s << "  if (threadIdx.x == 0u) {\n    double t = 0.0;\n";
+    s << "    for (unsigned w = 0; w < " << bdim_ << " / 64u; ++w) t += csmi_lw[w];\n";
+    s << "    partials[" << (bdim_ == "blockDim.x" ? "blockIdx.x" : "tile") << "] = t;\n  }\n";
   }
 
   // Non-temporal stores also write through at device scope (`sc1 nt`, a vector store): the
@@ -856,7 +891,8 @@ namespace {
 // profiles tell expressions and configurations apart.
 std::string kernel_name(const std::string &key, const Options &o) {
   char buf[64];
-  std::snprintf(buf, sizeof buf, "csmi_fused_%016llx_v%du%d", static_cast<unsigned long long>(fnv1a(key)), o.V, o.U);
+  std::snprintf(buf, sizeof buf, "csmi_fused_%016llx_v%du%d%s", static_cast<unsigned long long>(fnv1a(key)), o.V, o.U,
+                o.sum ? "_sum" : "");
   return buf;
 }
 
@@ -946,6 +982,66 @@ BatchPlan plan_batch(const std::vector<BatchExpr> &members, int block, int math_
 
 }  // namespace
 
+static cs_status compile_fused(cs_device *device, const cs_node *nodes, int nnodes, const int32_t *outputs,
+                               int noutputs, int out_rank, const int32_t *out_shape,
+                               const cs_fused_options *options, bool sum, cs_fused **fused) {
+  return guard([&] {
+    if (!device) fail(CS_INVALID_DEVICE, "device is null");
+    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
+    Options o;
+    std::string key;
+    Dag dag = canonicalize(nodes, nnodes, outputs, noutputs, out_rank, out_shape);
+    o = resolve_options(options, dag);
+    o.sum = sum;
+    if (sum && dag.outputs.size() != 1) fail(CS_INVALID_VALUE, "a fused sum takes one output");
+    key = canonical_key(dag, o);
+    {
+      std::lock_guard<std::recursive_mutex> l(device->mu);
+      auto it = device->fused_cache.find(key);
+      if (it != device->fused_cache.end()) {
+        it->second->refs++;
+        *fused = it->second;
+        return;
+      }
+    }
+    int64_t vectors = 0;
+    const std::string name = kernel_name(key, o);
+    std::string src = Emitter(dag, o).generate(&vectors, name);
+    std::string log;
+    auto mod = std::make_shared<ModuleRef>();
+    mod->ordinal = device->ordinal;
+    mod->module = build_module(device, src, math_options(o.math_mode), &log);
+    auto *f = new cs_fused();
+    f->device = device;
+    f->key = key;
+    f->hash = fnv1a(key);
+    f->source = src;
+    f->module = mod;
+    CS_HIP(hipModuleGetFunction(&f->function, mod->module, name.c_str()));
+    f->nleaves = dag.nleaves;
+    int64_t total = 1;
+    for (auto d : dag.out_shape) total *= d;
+    f->out_elements = total * static_cast<int64_t>(dag.outputs.size());
+    f->vectors = vectors;
+    f->block = static_cast<uint32_t>(o.block);
+    int64_t per_block = int64_t(o.block) * o.U;
+    f->grid = static_cast<uint32_t>((vectors + per_block - 1) / per_block);
+    f->dag = std::move(dag);
+    f->opts = o;
+    std::lock_guard<std::recursive_mutex> l(device->mu);
+    auto it = device->fused_cache.find(key);
+    if (it != device->fused_cache.end()) {  // another thread won the race
+      delete f;
+      it->second->refs++;
+      *fused = it->second;
+      return;
+    }
+    f->refs = 2;  // one for the cache, one for the caller
+    device->fused_cache.emplace(key, f);
+    *fused = f;
+  });
+}
+
 extern "C" {
 
 cs_status cs_fused_generate_source(const cs_node *nodes, int nnodes, const int32_t *outputs,
@@ -1028,60 +1124,44 @@ cs_status cs_program_check(const char *source, const char *options, char **log) 
   return s;
 }
 
-cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes,
-                           const int32_t *outputs, int noutputs, int out_rank,
-                           const int32_t *out_shape, const cs_fused_options *options,
+cs_status cs_fused_compile(cs_device *device, const cs_node *nodes, int nnodes, const int32_t *outputs,
+                           int noutputs, int out_rank, const int32_t *out_shape, const cs_fused_options *options,
                            cs_fused **fused) {
+  return compile_fused(device, nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, false, fused);
+}
+
+cs_status cs_fused_compile_sum(cs_device *device, const cs_node *nodes, int nnodes, const int32_t *outputs,
+                               int noutputs, int out_rank, const int32_t *out_shape,
+                               const cs_fused_options *options, cs_fused **fused) {
+  return compile_fused(device, nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, true, fused);
+}
+
+cs_status cs_fused_sum(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves, float *out,
+                       double *out64) {
   return guard([&] {
-    if (!device) fail(CS_INVALID_DEVICE, "device is null");
-    if (!fused) fail(CS_INVALID_VALUE, "fused is null");
-    Options o;
-    std::string key;
-    Dag dag = prepare(nodes, nnodes, outputs, noutputs, out_rank, out_shape, options, &o, &key);
-    {
-      std::lock_guard<std::recursive_mutex> l(device->mu);
-      auto it = device->fused_cache.find(key);
-      if (it != device->fused_cache.end()) {
-        it->second->refs++;
-        *fused = it->second;
-        return;
+    if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (!fused->opts.sum) fail(CS_INVALID_KERNEL_ARGS, "the kernel was not compiled by cs_fused_compile_sum");
+    if (!out && !out64) fail(CS_INVALID_VALUE, "no output");
+    if (nleaves != fused->nleaves)
+      fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
+                                       std::to_string(nleaves));
+    set_device(stream->device);
+    double *partials = sum_partials(stream, std::max<size_t>(fused->grid, 1));
+    if (fused->grid > 0) {
+      std::vector<const void *> vals(static_cast<size_t>(nleaves) + 1);
+      std::vector<void *> args(static_cast<size_t>(nleaves) + 1);
+      for (int i = 0; i < nleaves; ++i) {
+        if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
+        vals[i] = leaves[i];
+        args[i] = &vals[i];
       }
+      vals[nleaves] = partials;
+      args[nleaves] = &vals[nleaves];
+      CS_HIP(hipModuleLaunchKernel(fused->function, fused->grid, 1, 1, fused->block, 1, 1, 0, stream->stream,
+                                   args.data(), nullptr));
     }
-    int64_t vectors = 0;
-    const std::string name = kernel_name(key, o);
-    std::string src = Emitter(dag, o).generate(&vectors, name);
-    std::string log;
-    auto mod = std::make_shared<ModuleRef>();
-    mod->ordinal = device->ordinal;
-    mod->module = build_module(device, src, math_options(o.math_mode), &log);
-    auto *f = new cs_fused();
-    f->device = device;
-    f->key = key;
-    f->hash = fnv1a(key);
-    f->source = src;
-    f->module = mod;
-    CS_HIP(hipModuleGetFunction(&f->function, mod->module, name.c_str()));
-    f->nleaves = dag.nleaves;
-    int64_t total = 1;
-    for (auto d : dag.out_shape) total *= d;
-    f->out_elements = total * static_cast<int64_t>(dag.outputs.size());
-    f->vectors = vectors;
-    f->block = static_cast<uint32_t>(o.block);
-    int64_t per_block = int64_t(o.block) * o.U;
-    f->grid = static_cast<uint32_t>((vectors + per_block - 1) / per_block);
-    f->dag = std::move(dag);
-    f->opts = o;
-    std::lock_guard<std::recursive_mutex> l(device->mu);
-    auto it = device->fused_cache.find(key);
-    if (it != device->fused_cache.end()) {  // another thread won the race
-      delete f;
-      it->second->refs++;
-      *fused = it->second;
-      return;
-    }
-    f->refs = 2;  // one for the cache, one for the caller
-    device->fused_cache.emplace(key, f);
-    *fused = f;
+    launch_finish_sum(stream, partials, fused->grid, out, out64);  // zero partials: an empty sum, 0
+    CS_HIP(hipGetLastError());
   });
 }
 
@@ -1177,6 +1257,7 @@ cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *
                                 void *out, int block) {
   return guard([&] {
     if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (fused->opts.sum) fail(CS_INVALID_KERNEL_ARGS, "a sum kernel runs through cs_fused_sum");
     if (block == 0) block = static_cast<int>(fused->block);
     if (block < 64 || block > 1024 || block % 64)
       fail(CS_INVALID_WORK_GROUP_SIZE, "block must be a multiple of 64 in 64..1024");
@@ -1242,6 +1323,7 @@ cs_status cs_fused_launch(cs_stream *stream, cs_fused *fused, const void *const 
                           void *out) {
   return guard([&] {
     if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (fused->opts.sum) fail(CS_INVALID_KERNEL_ARGS, "a sum kernel runs through cs_fused_sum");
     if (nleaves != fused->nleaves)
       fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
                                        std::to_string(nleaves));
@@ -1266,6 +1348,7 @@ cs_status cs_fused_launch_timed(cs_stream *stream, cs_fused *fused, const void *
                                 void *out, cs_event **start, cs_event **stop) {
   return guard([&] {
     if (!stream || !fused) fail(CS_INVALID_VALUE, "null stream or kernel");
+    if (fused->opts.sum) fail(CS_INVALID_KERNEL_ARGS, "a sum kernel runs through cs_fused_sum");
     if (!start || !stop) fail(CS_INVALID_VALUE, "null event pointer");
     if (nleaves != fused->nleaves)
       fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
@@ -1341,11 +1424,13 @@ cs_status cs_batch_create(cs_device *device, cs_fused *const *exprs, int n, int 
     if (!device || !batch) fail(CS_INVALID_VALUE, "null argument");
     if (n <= 0 || !exprs) fail(CS_INVALID_VALUE, "empty batch");
     if (block < 64 || block > 1024 || block % 64) fail(CS_INVALID_WORK_GROUP_SIZE, "block must be 64..1024");
+    for (int i = 0; i < n; ++i)
+      if (!exprs[i]) fail(CS_INVALID_VALUE, "null expression");
     int math_mode = exprs[0]->opts.math_mode;
     for (int i = 0; i < n; ++i) {
-      if (!exprs[i]) fail(CS_INVALID_VALUE, "null expression");
       if (exprs[i]->device != device) fail(CS_INVALID_DEVICE, "expression compiled on another device");
       if (exprs[i]->opts.math_mode != math_mode) fail(CS_INVALID_VALUE, "mixed math modes in one batch");
+      if (exprs[i]->opts.sum) fail(CS_INVALID_KERNEL_ARGS, "a sum kernel cannot join a batch");
     }
     std::vector<BatchExpr> members;
     for (int i = 0; i < n; ++i) members.push_back({&exprs[i]->dag, exprs[i]->opts});

@@ -172,6 +172,10 @@ namespace csmi {
 // Stream-owned scratch of at least `bytes`, zero-filled when (re)allocated. Growing it
 // synchronizes the stream, so the first call of a given size must not be under capture.
 void *stream_scratch(cs_stream *stream, size_t bytes);
+// FP64 block partials of a sum in the stream's scratch (n of them), and the fixed-order FP64
+// combine that turns n partials into *out (FP32, nullable) and *out64 (nullable).
+double *sum_partials(cs_stream *stream, size_t n);
+void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, float *out, double *out64);
 // Events recorded now on every live stream of the device (for stream-ordered frees).
 std::vector<hipEvent_t> record_on_all_streams(cs_device *device);
 // Makes `device` current on the calling thread.

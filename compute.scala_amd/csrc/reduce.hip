@@ -284,6 +284,14 @@ static ReduceScratch scratch_for(cs_stream *stream, size_t partial_doubles) {
   return r;
 }
 
+namespace csmi {
+double *sum_partials(cs_stream *stream, size_t n) { return scratch_for(stream, n).partial64; }
+
+void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, float *out, double *out64) {
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, partials, n, out, out64);
+}
+}  // namespace csmi
+
 static int reduce_grid(cs_stream *stream, int64_t n) {
   int cus = stream->device->props.multiProcessorCount;
   int64_t want = (n / 4 + kBlock * 4 - 1) / (kBlock * 4);  // >= 4 vectors per thread

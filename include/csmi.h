@@ -265,6 +265,19 @@ typedef struct cs_fused_desc {
   cs_fused_options options;
 } cs_fused_desc;
 cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out);
+/* A reduction fused into its producer (the fusion barrier a `sum` puts after an inline
+ * tensor, Tensors.scala:673-771 over 1291-1392, removed): cs_fused_compile_sum builds the
+ * kernel of a single-output DAG that adds its elements (compensated FP32 per thread, FP64
+ * across the block) into FP64 block partials instead of storing them, and cs_fused_sum runs
+ * it and combines the partials in a fixed order: *out (FP32, nullable) and *out64 (FP64,
+ * nullable) get the sum of the values the element-wise kernel would have written, within
+ * 4 eps32 sum|x|, the same on every run. Such a handle is rejected by the element-wise
+ * launches and by batches. */
+cs_status cs_fused_compile_sum(cs_device *device, const cs_node *nodes, int nnodes, const int32_t *outputs,
+                               int noutputs, int out_rank, const int32_t *out_shape,
+                               const cs_fused_options *options, cs_fused **fused);
+cs_status cs_fused_sum(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves, float *out,
+                       double *out64);
 /* cs_fused_launch that also returns two new events (refcount 1) stamped with the kernel
  * dispatch's own start and end, as OpenCL's profiling info gives for a command
  * (CL_PROFILING_COMMAND_START/END; the reference maps CL_PROFILING_INFO_NOT_AVAILABLE at

@@ -170,3 +170,58 @@ def test_rows_of_empty_matrices(gpu_tensors):
     assert np.array_equal(download(T, rs, 3), np.zeros(3, np.float32)) and download(T, tot, 1)[0] == 0.0
     for h in (tot, tot64, rs):
         h.release()
+
+
+def test_fused_sum_of_an_inline_tensor():
+    """`(a*b + c).sum` runs as one kernel that computes the elements and adds them (no output
+    written): within 4*eps32*sum|x| of the FP64 sum of the FP32 element values, within
+    8*eps32*sum|x| of the unfused sum, the same bits on every run, one launch."""
+    from compute import Tensors
+    a, b, c = inputs.c1_inputs()
+    with Tensors(0) as T, Tensors(0, fused_sums=False) as U:
+        ta, tb, tc = (T.Tensor(x).doCache() for x in (a, b, c))
+        e = ta * tb + tc
+        before = T.launches
+        s1 = e.sum().flatArray()[0]
+        assert T.launches - before == 1
+        s2 = e.sum().flatArray()[0]
+        assert np.float32(s1).view(np.uint32) == np.float32(s2).view(np.uint32)
+        vals = ((a * b) + c).astype(np.float64)
+        bound = 4 * fp64.EPS32 * np.abs(vals).sum()
+        assert abs(float(s1) - vals.sum()) <= bound
+        ua, ub, uc = (U.Tensor(x) for x in (a, b, c))
+        s_unfused = (ua * ub + uc).sum().flatArray()[0]
+        assert abs(float(s1) - float(s_unfused)) <= 2 * bound
+        for t in (ta, tb, tc):
+            t.release()
+
+
+def test_fused_sum_of_views_and_edges(gpu_tensors):
+    """The view chain with exp and tanh, a broadcast fill, an empty tensor: fused sums agree
+    with the sum of the element-wise kernel's own output."""
+    T = gpu_tensors
+    X, bias = inputs.c2_inputs()
+    p = T.Tensor(X).permute([1, 0, 2])
+    y = p + T.Tensor(bias).broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+    out = T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+    vals = out.flatArray().astype(np.float64)
+    s = out.sum().flatArray()[0]
+    assert abs(float(s) - vals.sum()) <= 4 * fp64.EPS32 * np.abs(vals).sum()
+    assert T.Tensor.fill(15625.0, (8, 8)).sum().toString() == "1000000.0"
+    empty = T.Tensor(np.zeros((0, 5), np.float32))
+    assert (empty + empty).sum().flatArray()[0] == 0.0
+
+
+def test_sum_kernel_is_not_an_elementwise_kernel(gpu_tensors):
+    T = gpu_tensors
+    x = T.Tensor(np.ones((4, 4), np.float32))
+    f, leaves = T.compile_sum((x + x).closure, (4, 4))
+    try:
+        out, leaf = alloc(T, 16), leaves[0].id._buffer({})
+        ptrs = (ctypes.c_void_p * 1)(leaf.ptr)
+        with pytest.raises(capi.InvalidKernelArgs):
+            capi.check(capi.lib.cs_fused_launch(T.stream, f, ptrs, 1, out.ptr))
+        out.release()
+        leaf.release()
+    finally:
+        capi.lib.cs_fused_release(f)
