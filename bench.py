@@ -299,7 +299,7 @@ def bench_gemm(T, capi, variants, reps=10):
     return out
 
 
-def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=0):
+def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1):
     """512 uneven expressions: per-expression launches vs one persistent batch (both
     graph-captured), with configurations from the cost model when its parameters exist.
     With `sweep`, first runs the exhaustive autotuner over every configuration, trains the
@@ -467,9 +467,8 @@ def main():
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
     ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
-    ap.add_argument("--c5-nontemporal", type=int, default=0,
-                    help="c5: compile every expression with write-through non-temporal stores (1; measured "
-                         "slower on these small, uneven outputs: batch 1441 us against 957 us) or plain (0)")
+    ap.add_argument("--c5-nontemporal", type=int, default=1,
+                    help="c5: compile every expression with non-temporal stores (1) or plain stores (0)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")

@@ -447,11 +447,11 @@ class Emitter {
       if (run % 4 == 0 && run - i >= 4) {
         s << "    " << store("dst + " + std::to_string(i),
                              "csmi_v4{" + vals[i] + ", " + vals[i + 1] + ", " + vals[i + 2] + ", " + vals[i + 3] + "}",
-                             "csmi_v4")
+                             "csmi_v4", run)
           << "\n";
         i += 4;
       } else {
-        s << "    " << store("dst + " + std::to_string(i), vals[i], "float") << "\n";
+        s << "    " << store("dst + " + std::to_string(i), vals[i], "float", run) << "\n";
         i += 1;
       }
     }
@@ -478,11 +478,16 @@ class Emitter {
     s << "    partials[" << (bdim_ == "blockDim.x" ? "blockIdx.x" : "tile") << "] = t;\n  }\n";
   }
 
-  // Non-temporal stores also write through at device scope (`sc1 nt`, a vector store): the
-  // output is not read again, and no dirty line is left in L2 for the release at the end of
-  // the kernel, which otherwise writes it back between this launch and the next.
-  std::string store(const std::string &addr, const std::string &val, const std::string &type) {
+  // Non-temporal stores. When one store instruction covers the wave's lanes contiguously (a
+  // run of 1 or 4 floats per vector) they also write through at device scope (`sc1 nt`, a
+  // vector store): the output is not read again, and no dirty line is left in L2 for the
+  // release at the end of the kernel, which otherwise writes it back between this launch and
+  // the next. Longer or odd runs leave each instruction every other piece of a line, which
+  // written through becomes partial-line writes: those stay plain non-temporal stores.
+  std::string store(const std::string &addr, const std::string &val, const std::string &type, int run) {
     if (o_.nt) {
+      const bool whole = (run == 4 && type == "csmi_v4") || (run == 1 && type == "float");
+      if (!whole) return "__builtin_nontemporal_store(" + val + ", (" + gq_ + type + "*)(" + addr + "));";
       const std::string op = type == "csmi_v4" ? "global_store_dwordx4" : "global_store_dword";
       return "asm volatile(\"" + op + " %0, %1, off sc1 nt\" :: \"v\"((" + gq_ + type + "*)(" + addr +
              ")), \"v\"(" + val + ") : \"memory\");";

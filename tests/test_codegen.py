@@ -178,8 +178,9 @@ def test_batch_source_offline(offline_tensors):
 
 @pytest.mark.parametrize("v", [1, 2, 4, 8])
 def test_nontemporal_stores_write_through(v):
-    """With non-temporal stores the outputs leave through vector stores that write through at
-    device scope (`sc1 nt`): one 16-byte store per 4 floats of a run, single floats otherwise."""
+    """With non-temporal stores, a run of 4 floats per vector (one 16-byte store covering the
+    wave's lanes contiguously) writes through at device scope (`sc1 nt`); longer and odd runs
+    stay plain non-temporal stores (written through they would be partial-line writes)."""
     from compute import Tensors
     T = Tensors(offline=True, vector_width=v)
     T.options.nontemporal_store = 1
@@ -187,6 +188,10 @@ def test_nontemporal_stores_write_through(v):
     j = T.Tensor.join([x + x, x * x])  # two outputs: runs of 2V floats
     src = T.kernel_source(trees.Concatenate([p.closure for p in j._parts]), (3, 7, 8))
     run = 2 * v
-    assert src.count("global_store_dwordx4 %0, %1, off sc1 nt") == (run // 4 if run % 4 == 0 else 0)
-    assert src.count("global_store_dword %0, %1, off sc1 nt") == (0 if run % 4 == 0 else run)
+    assert src.count("off sc1 nt") == (1 if run == 4 else 0)
+    assert src.count("__builtin_nontemporal_store(") == (0 if run == 4 else (run // 4 if run % 4 == 0 else run))
+    compiles(src)
+    y = T.Tensor(np.zeros((3, 7, 8), np.float32))
+    src = T.kernel_source((y * y).closure, (3, 7, 8))  # one output: a run of V floats
+    assert ("off sc1 nt" in src) == (v in (1, 4))
     compiles(src)
