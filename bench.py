@@ -299,7 +299,7 @@ def bench_gemm(T, capi, variants, reps=10):
     return out
 
 
-def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1):
+def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1, batch_vu=(2, 2)):
     """512 uneven expressions: per-expression launches vs one persistent batch (both
     graph-captured), with configurations from the cost model when its parameters exist.
     With `sweep`, first runs the exhaustive autotuner over every configuration, trains the
@@ -399,8 +399,7 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     # odd), U = 2 and 512 threads: the batch's occupancy is set by its largest body, and these
     # bodies stay within 64 VGPRs (8 waves per SIMD), which measured faster than each
     # expression's own best standalone configuration.
-    batch_vu = (2, 2)
-    bsel = [fused[batch_vu][i] for i in range(len(exprs))]
+    bsel = [fused[tuple(batch_vu)][i] for i in range(len(exprs))]
     t0 = time.time()
     b = bm.Batch(eset, bsel, block=512)
     res["batch_compile_s"] = time.time() - t0
@@ -467,6 +466,7 @@ def main():
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
     ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
+    ap.add_argument("--c5-batch", default="2,2", help="c5: vector width and items per thread of the batch bodies")
     ap.add_argument("--c5-nontemporal", type=int, default=1,
                     help="c5: compile every expression with non-temporal stores (1) or plain stores (0)")
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -513,7 +513,8 @@ def main():
             g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
     if "c5" in only:
         also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir,
-                                           nontemporal=args.c5_nontemporal)
+                                           nontemporal=args.c5_nontemporal,
+                                           batch_vu=tuple(int(x) for x in args.c5_batch.split(",")))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline_c2()
