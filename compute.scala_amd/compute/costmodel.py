@@ -5,7 +5,7 @@ launch of a fused kernel from features of its expression DAG and its launch conf
 (vector width V, vectors per thread U, workgroup size). It is trained on the device by the
 library's own forward, backward and Adam kernels (cs_mlp_*) on timings that the exhaustive
 autotuner collects, and it picks each expression's configuration; exhaustive search over the
-same candidates remains the fallback and the judge of the model's regret.
+same candidates remains the fallback and the measure of the model's regret.
 """
 import ctypes
 import math
