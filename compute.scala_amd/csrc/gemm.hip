@@ -230,60 +230,93 @@ __device__ __forceinline__ int fp8_scale_exponent(float amax) {
   }
   return e < -127 ? -127 : (e > 127 ? 127 : e);
 }
-__device__ __forceinline__ uint8_t fp8_e4m3(float x) {
-  return static_cast<uint8_t>(__builtin_amdgcn_cvt_pk_fp8_f32(x, 0.0f, 0, false) & 0xff);
+
+// Four e4m3 bytes of x[0..3] * inv, packed into one word (byte i = value i).
+__device__ __forceinline__ uint32_t fp8x4(float x0, float x1, float x2, float x3, float inv) {
+  int w = __builtin_amdgcn_cvt_pk_fp8_f32(x0 * inv, x1 * inv, 0, false);
+  w = __builtin_amdgcn_cvt_pk_fp8_f32(x2 * inv, x3 * inv, w, true);
+  return static_cast<uint32_t>(w);
 }
 
-// A (rows x cols, K contiguous) -> X[rows_p][kp] e4m3 plus scales[kb * rows_p + r]: one wave
-// per 128-wide block of a row, two consecutive values per lane.
+// A (rows x cols, K contiguous) -> X[rows_p][kp] e4m3 plus scales[kb * rows_p + r]: 16 lanes
+// per 128-wide block of a row, 8 consecutive values per lane (two 16-byte loads when the
+// source is 16-byte aligned, `vec`), one 8-byte store per lane.
 __global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__restrict__ src, int rows, int cols,
-                                                                int ld, int rows_p, int kp,
+                                                                int ld, int rows_p, int kp, int vec,
                                                                 uint8_t *__restrict__ q,
                                                                 uint8_t *__restrict__ scales) {
-  const int wave = (blockIdx.x * 256 + threadIdx.x) / 64, lane = threadIdx.x % 64;
+  const int64_t group = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 16;
+  const int l16 = threadIdx.x % 16;
   const int blocks = kp / 128;
-  if (wave >= rows_p * blocks) return;
-  const int r = wave / blocks, kb = wave % blocks, k = kb * 128 + 2 * lane;
+  if (group >= int64_t(rows_p) * blocks) return;  // whole groups of 16 lanes leave together
+  const int r = static_cast<int>(group / blocks), kb = static_cast<int>(group % blocks);
+  const int k = kb * 128 + 8 * l16;
   const float *p = src + int64_t(r) * ld + k;
-  const float x0 = (r < rows && k < cols) ? p[0] : 0.0f;
-  const float x1 = (r < rows && k + 1 < cols) ? p[1] : 0.0f;
-  float m = fmaxf(fabsf(x0), fabsf(x1));
-  for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+  float v[8];
+  if (vec && r < rows && k + 8 <= cols) {
+    const float4 a = *reinterpret_cast<const float4 *>(p), b = *reinterpret_cast<const float4 *>(p + 4);
+    v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (r < rows && k + e < cols) ? p[e] : 0.0f;
+  }
+  float m = 0.0f;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) m = fmaxf(m, fabsf(v[e]));
+#pragma unroll
+  for (int off = 8; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 16));
   const int e = fp8_scale_exponent(m);
   const float inv = ldexpf(1.0f, -e);
-  const uint16_t two = uint16_t(fp8_e4m3(x0 * inv) | (uint16_t(fp8_e4m3(x1 * inv)) << 8));
-  *reinterpret_cast<uint16_t *>(q + int64_t(r) * kp + k) = two;
-  if (lane == 0) scales[int64_t(kb) * rows_p + r] = static_cast<uint8_t>(e + 127);
+  const uint2 w = make_uint2(fp8x4(v[0], v[1], v[2], v[3], inv), fp8x4(v[4], v[5], v[6], v[7], inv));
+  *reinterpret_cast<uint2 *>(q + int64_t(r) * kp + k) = w;
+  if (l16 == 0) scales[int64_t(kb) * rows_p + r] = static_cast<uint8_t>(e + 127);
 }
 
 // B (K x N, N contiguous) -> Y[rows_p][kp] e4m3 (row n = column n of B) plus
-// scales[kb * rows_p + n]: a block transposes 128 K x 64 N through LDS, then each wave
-// quantises 16 columns, one 128-wide block each.
+// scales[kb * rows_p + n]: a block stages 128 K x 64 N in LDS (16-byte loads along N when
+// `vec`), then 32 lanes quantise one column's 128-wide block, 4 consecutive K values per lane
+// and one 4-byte store each (the column's 128 bytes are contiguous in Y).
 __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__restrict__ src, int n_src,
-                                                                int k_src, int ld, int rows_p, int kp,
+                                                                int k_src, int ld, int rows_p, int kp, int vec,
                                                                 uint8_t *__restrict__ q,
                                                                 uint8_t *__restrict__ scales) {
-  __shared__ float tile[128][65];  // [k][n]
+  __shared__ float tile[128][65];  // [k][n]; row stride 65 words: lane l of a column read hits bank 4l + j + n
   const int n0 = blockIdx.y * 64, kb = blockIdx.x, k0 = kb * 128, t = threadIdx.x;
+  const int c4 = (t % 16) * 4;  // first of the four columns this thread loads
 #pragma unroll
-  for (int i = 0; i < 32; ++i) {
-    const int k = t / 64 + 4 * i, n = t % 64;
-    tile[k][n] = (k0 + k < k_src && n0 + n < n_src) ? src[int64_t(k0 + k) * ld + (n0 + n)] : 0.0f;
+  for (int i = 0; i < 8; ++i) {
+    const int k = t / 16 + 16 * i;
+    const float *p = src + int64_t(k0 + k) * ld + (n0 + c4);
+    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (k0 + k < k_src) {
+      if (vec && n0 + c4 + 4 <= n_src) {
+        a = *reinterpret_cast<const float4 *>(p);
+      } else {
+        a.x = n0 + c4 < n_src ? p[0] : 0.0f;
+        a.y = n0 + c4 + 1 < n_src ? p[1] : 0.0f;
+        a.z = n0 + c4 + 2 < n_src ? p[2] : 0.0f;
+        a.w = n0 + c4 + 3 < n_src ? p[3] : 0.0f;
+      }
+    }
+    tile[k][c4] = a.x;
+    tile[k][c4 + 1] = a.y;
+    tile[k][c4 + 2] = a.z;
+    tile[k][c4 + 3] = a.w;
   }
   __syncthreads();
-  const int wave = t / 64, lane = t % 64;
-  for (int j = 0; j < 16; ++j) {
-    const int n = wave * 16 + j;
+  const int l32 = t % 32, half = t / 32;  // 8 half-waves, each on one column at a time
+  for (int j = 0; j < 8; ++j) {
+    const int n = half + 8 * j;
     if (n0 + n >= rows_p) break;
-    const float x0 = tile[lane][n], x1 = tile[lane + 64][n];
-    float m = fmaxf(fabsf(x0), fabsf(x1));
-    for (int off = 32; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
+    const float x0 = tile[4 * l32][n], x1 = tile[4 * l32 + 1][n], x2 = tile[4 * l32 + 2][n],
+                x3 = tile[4 * l32 + 3][n];
+    float m = fmaxf(fmaxf(fabsf(x0), fabsf(x1)), fmaxf(fabsf(x2), fabsf(x3)));
+#pragma unroll
+    for (int off = 16; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 32));
     const int e = fp8_scale_exponent(m);
     const float inv = ldexpf(1.0f, -e);
-    uint8_t *row = q + int64_t(n0 + n) * kp + k0;
-    row[lane] = fp8_e4m3(x0 * inv);
-    row[lane + 64] = fp8_e4m3(x1 * inv);
-    if (lane == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
+    *reinterpret_cast<uint32_t *>(q + int64_t(n0 + n) * kp + k0 + 4 * l32) = fp8x4(x0, x1, x2, x3, inv);
+    if (l32 == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
   }
 }
 
@@ -624,10 +657,10 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
   if (KIND == kFP8) {
     sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes);
     sy = sx + Mp * (Kp / 128);
-    hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) + 3) / 4)), dim3(256), 0, s, A, M,
-                       K, lda, int(Mp), int(Kp), reinterpret_cast<uint8_t *>(x), sx);
+    hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) * 16 + 255) / 256)), dim3(256), 0, s,
+                       A, M, K, lda, int(Mp), int(Kp), vec_a, reinterpret_cast<uint8_t *>(x), sx);
     hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Kp / 128), unsigned(Np / 64)), dim3(256), 0, s, B, N,
-                       K, ldb, int(Np), int(Kp), reinterpret_cast<uint8_t *>(y), sy);
+                       K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<uint8_t *>(y), sy);
   } else if (KIND == kF32) {
     hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
                        lda, int(Mp), int(Kp), vec_a, reinterpret_cast<float *>(x));
