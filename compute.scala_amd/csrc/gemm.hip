@@ -64,13 +64,6 @@ __device__ __forceinline__ void tile_coords(int b, int tiles_m, int tiles_n, int
 }
 
 // ---------------------------------------------------------------------------- pre-pass
-__device__ __forceinline__ uint16_t bf16_rne(float x) {
-  uint32_t u = __float_as_uint(x);
-  if ((u & 0x7f800000u) == 0x7f800000u) return static_cast<uint16_t>((u >> 16) | ((u & 0xffff) ? 0x40 : 0));
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return static_cast<uint16_t>(u >> 16);
-}
-__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
 // FP32 operands for the FP32 kernel, zero padded to [rows_p][kp]. A (K contiguous): one
 // thread per 4 consecutive K values of a row, 16-byte loads when the source is 16-byte
@@ -134,23 +127,28 @@ __global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restr
 // Operand rows for the LDS pipeline. BF16: Kp values per row. Split (BF16X3): per 32-wide K
 // block, 32 BF16 high parts then 32 BF16 low parts (128 bytes), so that one K stage of a row
 // is 128 contiguous bytes in every variant.
+// Two floats rounded to BF16 (nearest even) by v_cvt_pk_bf16_f32: a in the low half.
+__device__ __forceinline__ uint32_t bf16x2(float a, float b) {
+  typedef float f32x2 __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(f32x2{a, b}, bf16x2_t));
+}
+
 __device__ __forceinline__ void store_bf16x8(const float (&v)[8], char *row, int k, bool split) {
-  uint16_t hi[8], lo[8];
+  i32x4 hi, lo;
 #pragma unroll
-  for (int e = 0; e < 8; ++e) {
-    hi[e] = bf16_rne(v[e]);
-    lo[e] = bf16_rne(v[e] - bf16_to_f32(hi[e]));
+  for (int e = 0; e < 4; ++e) {
+    const uint32_t h = bf16x2(v[2 * e], v[2 * e + 1]);
+    hi[e] = int(h);
+    // the low parts: each value minus its high part (exact), rounded once more
+    lo[e] = int(bf16x2(v[2 * e] - __uint_as_float(h << 16), v[2 * e + 1] - __uint_as_float(h & 0xffff0000u)));
   }
-  auto pack = [](const uint16_t *h) {
-    return i32x4{int(h[0] | (uint32_t(h[1]) << 16)), int(h[2] | (uint32_t(h[3]) << 16)),
-                 int(h[4] | (uint32_t(h[5]) << 16)), int(h[6] | (uint32_t(h[7]) << 16))};
-  };
   if (split) {
     char *p = row + (k / 32) * 128 + (k % 32) * 2;
-    *reinterpret_cast<i32x4 *>(p) = pack(hi);
-    *reinterpret_cast<i32x4 *>(p + 64) = pack(lo);
+    *reinterpret_cast<i32x4 *>(p) = hi;
+    *reinterpret_cast<i32x4 *>(p + 64) = lo;
   } else {
-    *reinterpret_cast<i32x4 *>(row + int64_t(k) * 2) = pack(hi);
+    *reinterpret_cast<i32x4 *>(row + int64_t(k) * 2) = hi;
   }
 }
 
