@@ -454,6 +454,52 @@ def cpu_baseline_c2(seconds=10.0):
                       "no FMA) in %.1f s" % (runs, el)}
 
 
+def _timed(fn, seconds):
+    """Calls fn until about `seconds` have passed (at least twice); returns (calls, elapsed)."""
+    fn()
+    runs, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        runs += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and runs >= 2:
+            return runs, el
+
+
+def cpu_reference_other(seconds=3.0):
+    """The reference's CPU programs for the other workloads, each on a bounded sample on this
+    host: the elementwise kernel text of C1 on all threads, the CPU device's sequential `sum`
+    program (one thread, as the reference launches it), the row folds that `split(1).reduce`
+    unrolls to, and the sequential-k FP32 product of the matrixMultiply idiom."""
+    import oracle
+    from compute import Tensors
+    from oracle import inputs, refcl
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    res = {}
+    T = Tensors(offline=True)
+    a, b, c = inputs.c1_inputs()
+    e = T.Tensor(a) * T.Tensor(b) + T.Tensor(c)
+    k = refcl.Kernel(e.closure, e.shape, "precise")
+    runs, el = _timed(lambda: k([a, b, c]), seconds)
+    res["c1_fma"] = {"value": round(16 * a.size * runs / el / 1e9, 3), "unit": "GB/s", "cores": threads,
+                     "sample": "%d launches of a*b+c [1024,1024] (reference kernel text, host C)" % runs}
+    x = ((a * b) + c).ravel()
+    runs, el = _timed(lambda: oracle.sequential_sum(x), seconds)
+    res["c1_sum"] = {"value": round(4 * x.size * runs / el / 1e9, 3), "unit": "GB/s", "cores": 1,
+                     "sample": "%d runs of the CPU device's sum program over 2^20 floats" % runs}
+    rows, cols = 4096, 4096  # a quarter of C3's rows: same row length
+    m = inputs.normal(rows * cols, 3001).reshape(rows, cols)
+    runs, el = _timed(lambda: oracle.row_fold(m), seconds)
+    res["c3_rows"] = {"value": round(4 * m.size * runs / el / 1e9, 3), "unit": "GB/s", "cores": threads,
+                      "sample": "%d row folds (sum and max) of [4096,4096] (C3's row length)" % runs}
+    n = 512
+    ga, gb = inputs.uniform(n * n, 4001).reshape(n, n), inputs.uniform(n * n, 4002).reshape(n, n)
+    runs, el = _timed(lambda: oracle.gemm_seqk(ga, gb), seconds)
+    res["c4_gemm_fp32"] = {"value": round(2.0 * n ** 3 * runs / el / 1e12, 4), "unit": "TFLOP/s", "cores": threads,
+                           "sample": "%d sequential-k FP32 products at 512^3 (the idiom's order)" % runs}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -519,6 +565,8 @@ def main():
     if not args.no_cpu:
         cpu = cpu_baseline_c2()
         log("cpu baseline: %s" % cpu)
+        also["cpu_reference_other"] = cpu_reference_other()
+        log("cpu reference, other workloads: %s" % also["cpu_reference_other"])
     gbs = c2["gbs"]
     kernel_gbs = c2["bytes_per_launch"] / (c2["us_per_launch_events"] * 1e-6) / 1e9
     # HBM bytes per launch of the headline kernel, from the FETCH_SIZE / WRITE_SIZE passes
