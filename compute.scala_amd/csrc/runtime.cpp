@@ -286,7 +286,10 @@ static void cache_store(const std::string &path, const std::vector<char> &code) 
   if (path.empty()) return;
   std::string dir = path.substr(0, path.rfind('/'));
   ::mkdir(dir.c_str(), 0755);
-  std::string tmp = path + ".tmp" + std::to_string(::getpid());
+  // unique per writer (process and call), so that threads compiling the same kernel at once
+  // never write one temporary file together; the rename publishes a complete file
+  static std::atomic<unsigned> writes{0};
+  std::string tmp = path + ".tmp" + std::to_string(::getpid()) + "." + std::to_string(writes.fetch_add(1));
   {
     std::ofstream f(tmp, std::ios::binary);
     if (!f) return;

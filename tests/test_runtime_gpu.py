@@ -240,3 +240,34 @@ def test_timed_launch_matches_and_times(gpu_tensors):
     out1.release()
     out2.release()
     t.release()
+
+
+def test_threads_share_one_device():
+    """Eight host threads, each with its own Tensors (its own stream) on GPU 0, compile the
+    same and different expressions at once (the structural cache's race path), allocate and
+    release through the shared arena and read back: every result bit-exact."""
+    import threading
+    from compute import Tensors
+    from oracle import inputs
+    errors = []
+
+    def work(i):
+        try:
+            with Tensors(0) as T:
+                x = inputs.uniform(64 * 4096, 100 + i).reshape(64, 4096)
+                for rep in range(6):
+                    t = T.Tensor(x)
+                    same = (t * t + t).flatArray()  # one structure in every thread
+                    own = (t * T.Tensor.fill(float(i + rep), x.shape)).flatArray()  # a literal per thread
+                    assert np.array_equal(same.view(np.uint32), ((x * x) + x).ravel().view(np.uint32))
+                    assert np.array_equal(own.view(np.uint32), (x * np.float32(i + rep)).ravel().view(np.uint32))
+        except Exception as ex:  # collected and asserted in the main thread
+            errors.append((i, repr(ex)))
+
+    threads = [threading.Thread(target=work, args=(i,)) for i in range(8)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=100)
+    assert not any(th.is_alive() for th in threads)
+    assert not errors, errors
