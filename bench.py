@@ -334,14 +334,15 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
         res["timings"] = len(times)
         log("c5: %d autotuner timings in %.1f s" % (len(times), time.time() - t0))
         ntrain = (3 * len(exprs)) // 4
-        X, y = [], []
+        X, y, grp = [], [], []
         for (i, ci), us in times.items():
             if i < ntrain:
                 X.append(cm.features(exprs[i], cfgs[ci]))
                 y.append(us)
+                grp.append(i)
         model = cm.CostModel(T, len(X[0]))
         t0 = time.time()
-        hist = model.fit(np.stack(X), np.array(y), epochs=150, batch=512)
+        hist = model.fit(np.stack(X), np.array(y), epochs=300, batch=512, groups=np.array(grp))
         res["train_s"] = time.time() - t0
         res["train_loss_first_last"] = [hist[0], hist[-1]]
         best_t = model_t = default_t = 0.0
@@ -356,22 +357,30 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
         res["regret_vs_exhaustive"] = model_t / best_t - 1.0
         res["gain_over_default"] = default_t / model_t
         res["held_out_us"] = {"exhaustive": best_t, "model": model_t, "default": default_t}
-        log("c5: cost model regret %.3f, gain over default %.3f" % (res["regret_vs_exhaustive"],
-                                                                   res["gain_over_default"]))
+        # among the 256-thread configurations, the set the per-expression launches choose from
+        b256 = [c for c in cfgs if c["block_size"] == 256]
+        bb = bm_t = 0.0
+        for i in range(ntrain, len(exprs)):
+            tb = [times[(i, cfgs.index(c))] for c in b256]
+            bb += min(tb)
+            bm_t += times[(i, cfgs.index(model.choose(exprs[i], b256)))]
+        res["regret_256_threads"] = bm_t / bb - 1.0
+        res["default_regret_256_threads"] = default_t / bb - 1.0
+        log("c5: cost model regret %.3f, gain over default %.3f; among 256-thread configurations regret %.3f "
+            "(default %.3f)" % (res["regret_vs_exhaustive"], res["gain_over_default"], res["regret_256_threads"],
+                                res["default_regret_256_threads"]))
         os.makedirs(os.path.dirname(model_path), exist_ok=True)
-        np.savez(model_path, params=model.net.get_params(), mean=model.mean, std=model.std,
-                 ymean=np.array([model.ymean]), widths=np.array(model.net.widths))
+        for path in [model_path] + ([os.path.join(out_dir, "costmodel.npz")] if out_dir else []):
+            np.savez(path, params=model.net.get_params(), mean=model.mean, std=model.std,
+                     ymean=np.array([model.ymean]), widths=np.array(model.net.widths))
         if out_dir:
             with open(os.path.join(out_dir, "c5_timings.json"), "w") as f:
                 json.dump({"configs": cfgs, "times": [[i, ci, us] for (i, ci), us in sorted(times.items())]}, f)
         model.release()
     # configuration per expression: cost model if trained parameters exist, else default
     chosen = [default] * len(exprs)
-    if os.path.exists(model_path):
-        d = np.load(model_path)
-        model = cm.CostModel(T, int(d["widths"][0]), hidden=int(d["widths"][1]), layers=len(d["widths"]) - 2)
-        model.net.set_params(d["params"])
-        model.mean, model.std, model.ymean = d["mean"], d["std"], float(d["ymean"][0])
+    model = cm.load(T, model_path)
+    if model is not None:
         b256 = [c for c in cfgs if c["block_size"] == 256]
         chosen = [model.choose(e, b256) for e in exprs]
         model.release()

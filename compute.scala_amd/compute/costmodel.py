@@ -9,6 +9,7 @@ same candidates remains the fallback and the measure of the model's regret.
 """
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -100,7 +101,8 @@ class Network(object):
 
 
 class CostModel(object):
-    """Predicts log-microseconds; selects the fastest predicted configuration."""
+    """Predicts log-microseconds (relative to the expression's mean when trained with groups);
+    selects the fastest predicted configuration."""
 
     def __init__(self, tensors, nfeatures, hidden=64, layers=3, seed=1):
         self.T = tensors
@@ -116,10 +118,20 @@ class CostModel(object):
         check(lib.cs_buffer_write(self.T.stream, b.handle, 0, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1))
         return b
 
-    def fit(self, X, us, epochs=300, batch=512, lr=2e-3, seed=0):
-        """Trains on features X[n, f] and measured microseconds; returns the loss history."""
+    def fit(self, X, us, epochs=300, batch=512, lr=2e-3, seed=0, groups=None):
+        """Trains on features X[n, f] and measured microseconds; returns the loss history.
+        With `groups` (the expression of each row), the target is each time's log relative to
+        the mean log-time of its expression: the choice among one expression's configurations
+        depends only on those differences, and the spread of sizes across expressions no
+        longer dominates the loss."""
         X = np.asarray(X, np.float32)
-        y = np.log(np.asarray(us, np.float64)).astype(np.float32)
+        y = np.log(np.asarray(us, np.float64))
+        if groups is not None:
+            groups = np.asarray(groups)
+            for g in np.unique(groups):
+                sel = groups == g
+                y[sel] -= y[sel].mean()
+        y = y.astype(np.float32)
         self.mean = X.mean(axis=0)
         self.std = X.std(axis=0) + 1e-3
         self.ymean = float(y.mean())
@@ -158,6 +170,21 @@ class CostModel(object):
 
     def release(self):
         self.net.release()
+
+
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "costmodel.npz")
+
+
+def load(tensors, path=MODEL_PATH):
+    """The trained model saved by the exhaustive sweep (parameters and feature scaling), on
+    `tensors`'s device; None when no parameters were saved."""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path)
+    model = CostModel(tensors, int(d["widths"][0]), hidden=int(d["widths"][1]), layers=len(d["widths"]) - 2)
+    model.net.set_params(d["params"])
+    model.mean, model.std, model.ymean = d["mean"], d["std"], float(d["ymean"][0])
+    return model
 
 
 def measure(eset, fused_by_variant, cfgs, reps=3, min_us=40.0, progress=None):
