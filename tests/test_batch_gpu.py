@@ -23,8 +23,10 @@ def test_batch_matches_per_expression_launches(small_set):
     eset = small_set
     T = eset.T
     rng = np.random.default_rng(0)
+    # non-temporal stores on some bodies: a run of 4 (or 1) floats writes through at device
+    # scope, through the batch's global-address-space pointers
     cfgs = [{"vector_width": int(rng.choice([1, 2, 4])), "items_per_thread": int(rng.choice([1, 2, 4])),
-             "block_size": 256} for _ in eset.exprs]
+             "block_size": 256, "nontemporal_store": int(rng.integers(0, 2))} for _ in eset.exprs]
     fused = bm.compile_many(T, eset.exprs, cfgs)
     for i, f in enumerate(fused):
         eset.launch(i, f, int(rng.choice([64, 256, 1024])))
