@@ -15,7 +15,7 @@ the one-pass row sum/row max/total over [16384,4096], and C = A.B at 4096^3 per 
 variant with its FP64 error. `cpu_baseline` times the reference's generated kernel for the
 headline workload (its OpenCL C text run as host C on OpenMP threads) on a bounded sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c1,c3,c4]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--only c2,c1,c3,c4,conv,c5]
 """
 import argparse
 import ctypes
@@ -437,6 +437,60 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     return res
 
 
+def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
+    """The convolution idiom of the reference's benchmark at its largest setting
+    (benchmarks.scala:454-548, 612-631: input [128,32,32,8], 3x3 kernel, 8 filters, one
+    layer): a join of 8 per-filter sums of 72 products of translated input channels and
+    broadcast weights, one generated kernel, one element per thread (V=1, 27 us against
+    29 us at V=2 and 56 us at V=4: the kernel holds hundreds of values per element).
+    Reported separately (not a headline workload)."""
+    from compute import Tensors
+    from compute import tensors as tz, trees, workloads
+    T = Tensors(0, math_mode=T_outer.options.math_mode, vector_width=vector_width, block_size=256)
+    lib = capi.lib
+    B, H, W, D, F = 128, 32, 32, 8, 8
+    n_in, n_w, n_out = B * H * W * D, 3 * 3 * D * F, B * H * W * F
+    xs = [T._alloc(n_in * 4) for _ in range(sets)]
+    outs = [T._alloc(n_out * 4) for _ in range(sets)]
+    wb, bb = T._alloc(n_w * 4), T._alloc(F * 4)
+    for i in range(sets):
+        capi.check(lib.cs_fill_normal(T.stream, xs[i].ptr, n_in, 6001 + i))
+    capi.check(lib.cs_fill_normal(T.stream, wb.ptr, n_w, 6101))
+    capi.check(lib.cs_fill_normal(T.stream, bb.ptr, F, 6102))
+    X = tz.CachedTensor(T, (B, H, W, D), 0.0, xs[0].retain())
+    Wt = tz.CachedTensor(T, (3, 3, D, F), 0.0, wb.retain())
+    Bt = tz.CachedTensor(T, (F,), 0.0, bb.retain())
+    out = workloads.convolute(T, X, Wt, Bt)
+    clo = trees.Concatenate([p.closure for p in out._parts])
+    t0 = time.time()
+    f, leaves, nout = T.compile(clo, (B, H, W))
+    compile_s = time.time() - t0
+    order = {id(X): None, id(Wt): wb, id(Bt): bb}
+    ptrs = []
+    for i in range(sets):
+        order[id(X)] = xs[i]
+        ptrs.append((ctypes.c_void_p * len(leaves))(*[order[id(l.id)].ptr for l in leaves]))
+    launch = lambda i: capi.check(lib.cs_fused_launch(T.stream, f, ptrs[i], len(leaves), outs[i].ptr))
+    g = capture(capi, T.stream, [lambda i=i: launch(i) for i in range(sets)])
+    timer = Timer(capi, T.stream)
+    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), 4)
+    reps = max(1, steps // sets)
+    ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
+    us = ms * 1e3 / (reps * sets)
+    flop = 2 * B * H * W * F * 9 * D + B * H * W * F  # products and adds per output, plus the bias
+    res = {"us_per_forward": us, "gflops": flop / (us * 1e-6) / 1e9,
+           "gbs": 4 * (n_in + n_w + F + n_out) / (us * 1e-6) / 1e9, "compile_s": compile_s,
+           "config": "input [128,32,32,8], weight [3,3,8,8], bias [8], one layer, V=%d" % vector_width}
+    lib.cs_graph_release(g)
+    lib.cs_fused_release(f)
+    for t in (X, Wt, Bt):
+        t.release()
+    for h in xs + outs + [wb, bb]:
+        h.release()
+    T.close()
+    return res
+
+
 def cpu_baseline_c2(seconds=10.0):
     """The reference's kernel for the headline expression (refcl-host, precise mode) on all
     host cores, timed on whole launches until about `seconds` have passed."""
@@ -515,7 +569,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="c2,c1,c3,c4,c5")
+    ap.add_argument("--only", default="c2,c1,c3,c4,conv,c5")
     ap.add_argument("--c5-sweep", action="store_true", help="run the exhaustive autotuner and train the cost model")
     ap.add_argument("--out-dir", default=None)
     ap.add_argument("--c2-config", default=None,
@@ -566,6 +620,9 @@ def main():
         g = also["c4_gemm_4096"]
         if "bf16x3" in g and "fp32" in g:
             g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
+    if "conv" in only:
+        also["convolution_idiom"] = bench_conv(T, capi)
+        log("convolution idiom: %s" % also["convolution_idiom"])
     if "c5" in only:
         also["c5_uneven_batch"] = bench_c5(T, capi, sweep=args.c5_sweep, out_dir=args.out_dir,
                                            nontemporal=args.c5_nontemporal,

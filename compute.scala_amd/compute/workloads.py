@@ -123,3 +123,27 @@ def uneven_dags(T, n=512, seed=5001, min_elements=1_000, max_elements=4_000_000)
         exprs.append(Expr(e, expr, [leaves[i] for i in used], [data[i] for i in used], ops,
                           [kinds[i] for i in used]))
     return exprs
+
+
+def convolute(T, inp, weight, bias):
+    """The convolution of the reference's benchmark (benchmarks.scala:464-548), written with
+    the Tensor API as a user would: input [batch, height, width, depth], weight [kh, kw,
+    depth, filters], bias [filters]; each output channel is the bias plus the sum, in the
+    fold's order, of every translated input channel times its weight broadcast. Returns the
+    join of the channels ([batch, height, width, filters])."""
+    from functools import reduce
+    batch, height, width, depth = inp.shape
+    kh, kw, depth2, filters = weight.shape
+    if depth2 != depth or tuple(bias.shape) != (filters,):
+        raise ValueError("convolute: shapes do not agree")
+    inputs = list(inp.split(3))
+    weights = [[[list(d.split(0)) for d in kwd.split(0)] for kwd in khkwd.split(0)] for khkwd in weight.split(3)]
+    channels = []
+    for per_filter, b in zip(weights, bias.split(0)):
+        terms = []
+        for dy, per_row in zip(range(-(kh // 2), kh - kh // 2), per_filter):
+            for dx, per_pixel in zip(range(-(kw // 2), kw - kw // 2), per_row):
+                for channel, wt in zip(inputs, per_pixel):
+                    terms.append(channel.translate([0, dy, dx]) * wt.broadcast((batch, height, width)))
+        channels.append(b.broadcast((batch, height, width)) + reduce(lambda x, y: x + y, terms))
+    return T.Tensor.join(channels)

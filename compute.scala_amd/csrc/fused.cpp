@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <sstream>
+#include <unordered_map>
 
 #include <hip/hip_ext.h>
 
@@ -105,6 +106,33 @@ Dag canonicalize(const cs_node *nodes, int nnodes, const int32_t *outputs, int n
 
   std::vector<int> canon(static_cast<size_t>(nnodes), -1);
   int next_leaf = 0;
+  // Value numbering: a node equal to an earlier canonical node (same op, canonical operands
+  // and attributes) becomes that node, so structurally equal subtrees built as separate
+  // objects (the same view taken once per term, as the convolution idiom does) are gathered
+  // and computed once. The expressions are pure, so results are unchanged.
+  std::unordered_map<std::string, int> seen;
+  auto value_key = [&](const cs_node &c) {
+    std::string k;
+    put(k, c.op);
+    put(k, c.a);
+    put(k, c.b);
+    if (c.op == CS_OP_PARAMETER || c.op == CS_OP_LITERAL) {
+      uint32_t bits;
+      std::memcpy(&bits, &c.value, 4);
+      put(k, bits);
+    }
+    if (c.op == CS_OP_PARAMETER) put(k, c.param);
+    if (c.op == CS_OP_PARAMETER || c.op == CS_OP_TRANSFORM) {
+      put(k, c.rank);
+      for (int i = 0; i < c.rank; ++i) put(k, c.shape[i]);
+    }
+    if (c.op == CS_OP_TRANSFORM) {
+      const cs_node &opnd = dag.nodes[static_cast<size_t>(c.a)];
+      const int rows = (opnd.op == CS_OP_PARAMETER || opnd.op == CS_OP_TRANSFORM) ? opnd.rank : 0;
+      for (int i = 0; i < rows * (c.rank + 1); ++i) put(k, c.matrix[i]);
+    }
+    return k;
+  };
   // Iterative DFS: pre-order for leaf numbering, post-order for emission.
   struct Frame { int node; int stage; };
   for (int oi = 0; oi < noutputs; ++oi) {
@@ -148,8 +176,15 @@ Dag canonicalize(const cs_node *nodes, int nnodes, const int32_t *outputs, int n
       if (is_binary(n.op)) { c.a = map(n.a); c.b = map(n.b); }
       else if (is_unary(n.op) || n.op == CS_OP_TRANSFORM || n.op == CS_OP_FILL || n.op == CS_OP_EXTRACT) { c.a = map(n.a); c.b = -1; }
       else { c.a = c.b = -1; }
-      canon[id] = static_cast<int>(dag.nodes.size());
-      dag.nodes.push_back(c);
+      const std::string vk = value_key(c);
+      auto hit = seen.find(vk);
+      if (hit != seen.end()) {
+        canon[id] = hit->second;
+      } else {
+        canon[id] = static_cast<int>(dag.nodes.size());
+        dag.nodes.push_back(c);
+        seen.emplace(vk, canon[id]);
+      }
       on_stack[id] = 0;
       stack.pop_back();
     }
@@ -315,6 +350,14 @@ std::string canonical_key(const Dag &dag, const Options &o) {
   return k;
 }
 
+// The largest workgroup a compiled expression accepts. Small bodies are compiled for any
+// size up to 1024 threads (so one compile serves every launch shape the autotuner tries);
+// that caps a thread at 128 VGPRs, which a large body (a convolution's hundreds of terms)
+// exceeds and spills. Those are compiled for their own workgroup size only, which gives them
+// up to 512 registers.
+constexpr size_t kLargeBody = 256;  // canonical nodes
+int launch_bound(const Dag &dag, const Options &o) { return dag.nodes.size() > kLargeBody ? o.block : 1024; }
+
 // ------------------------------------------------------------------------ code generation
 // Kernel shape: each thread owns U vectors of V consecutive innermost outputs (vector u of
 // thread t in block b is q = (b*U + u)*BLOCK + t, so a wave's accesses are contiguous).
@@ -330,7 +373,7 @@ class Emitter {
   std::string generate(int64_t *vectors_out, const std::string &name, bool with_prelude = true) {
     std::ostringstream s;
     header(s, with_prelude);
-    s << "extern \"C\" __global__ __launch_bounds__(1024) void " << name << "(";
+    s << "extern \"C\" __global__ __launch_bounds__(" << launch_bound(dag_, o_) << ") void " << name << "(";
     params(s);
     s << ") {\n";
     body(s, "blockIdx.x", vectors_out);
@@ -870,6 +913,7 @@ struct cs_fused {
   int64_t vectors = 0;       // number of V-wide output vectors
   uint32_t block = 256;
   uint32_t grid = 0;
+  int max_block = 1024;      // the kernel's launch bound
   Dag dag;                   // kept for batches, which re-emit the body
   Options opts;
 };
@@ -1029,6 +1073,7 @@ static cs_status compile_fused(cs_device *device, const cs_node *nodes, int nnod
     f->out_elements = total * static_cast<int64_t>(dag.outputs.size());
     f->vectors = vectors;
     f->block = static_cast<uint32_t>(o.block);
+    f->max_block = launch_bound(dag, o);
     int64_t per_block = int64_t(o.block) * o.U;
     f->grid = static_cast<uint32_t>((vectors + per_block - 1) / per_block);
     f->dag = std::move(dag);
@@ -1229,6 +1274,7 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
         f->out_elements = total * static_cast<int64_t>(dags[i].outputs.size());
         f->vectors = vectors[k];
         f->block = static_cast<uint32_t>(opts[i].block);
+        f->max_block = launch_bound(dags[i], opts[i]);
         int64_t per_block = int64_t(opts[i].block) * opts[i].U;
         f->grid = static_cast<uint32_t>((vectors[k] + per_block - 1) / per_block);
         f->dag = dags[i];
@@ -1266,6 +1312,9 @@ cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *
     if (block == 0) block = static_cast<int>(fused->block);
     if (block < 64 || block > 1024 || block % 64)
       fail(CS_INVALID_WORK_GROUP_SIZE, "block must be a multiple of 64 in 64..1024");
+    if (block > fused->max_block)
+      fail(CS_INVALID_WORK_GROUP_SIZE, "this expression's kernel was compiled for at most " +
+                                           std::to_string(fused->max_block) + " threads per workgroup");
     if (nleaves != fused->nleaves)
       fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
                                        std::to_string(nleaves));

@@ -287,7 +287,9 @@ cs_status cs_fused_launch_timed(cs_stream *stream, cs_fused *fused, const void *
                                 void *out, cs_event **start, cs_event **stop);
 /* cs_fused_launch with the workgroup size chosen at launch (0 = the compiled default):
  * the generated code reads the workgroup size at run time, so one compile serves every
- * size the autotuner tries. */
+ * size the autotuner tries. A large expression (more than 256 nodes after merging equal
+ * subtrees) is compiled for its own workgroup size only, so that a thread may hold more
+ * registers: a larger block is rejected with CS_INVALID_WORK_GROUP_SIZE. */
 cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,
                                 void *out, int block);
 /* Compiles the fused kernel into the on-disk code-object cache without a device, so that a

@@ -228,3 +228,35 @@ def test_output_near_the_index_limit():
             assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), start
         y.release()
         x.release()
+
+
+def test_large_body_keeps_its_workgroup_size(gpu_tensors):
+    """An expression of more than 256 nodes is compiled for its own workgroup size (so that a
+    thread may hold up to 512 registers): launching it with a larger block is an error, with
+    its own size or smaller it computes the same result."""
+    import ctypes
+    from functools import reduce
+    from compute import _capi as capi
+    T = gpu_tensors
+    x = inputs.normal(64 * 64, 71).reshape(64, 64)
+    t = T.Tensor(x)
+    terms = [t.translate([0, k % 7 - 3]) * T.Tensor.fill(float(k + 1), (64, 64)) for k in range(150)]
+    e = reduce(lambda a, b: a + b, terms)
+    want = e.flatArray()
+    f, leaves, _ = T.compile(e.closure, e.shape)
+    try:
+        out = T._alloc(64 * 64 * 4)
+        src = leaves[0].id._buffer({})
+        ptrs = (ctypes.c_void_p * 1)(src.ptr)
+        with pytest.raises(capi.InvalidWorkGroupSize):
+            capi.check(capi.lib.cs_fused_launch_block(T.stream, f, ptrs, 1, out.ptr, 512))
+        for block in (256, 128):
+            capi.check(capi.lib.cs_fused_launch_block(T.stream, f, ptrs, 1, out.ptr, block))
+            got = np.empty(64 * 64, np.float32)
+            capi.check(capi.lib.cs_memcpy_dtoh(T.stream, got.ctypes.data_as(ctypes.c_void_p),
+                                               ctypes.c_void_p(out.ptr), got.nbytes))
+            assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), block
+        out.release()
+        src.release()
+    finally:
+        capi.lib.cs_fused_release(f)

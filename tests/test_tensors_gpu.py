@@ -98,3 +98,28 @@ def test_random_graphs_agree_with_oracle(gpu_tensors, seed):
 def _params(closure):
     from compute import trees
     return trees.parameters(closure)
+
+
+def test_convolution_idiom_bit_exact(gpu_tensors):
+    """The reference's convolution idiom (TensorsSpec.scala:144-210, benchmarks.scala:454-548:
+    a join of per-filter sums of translated input channels times broadcast weights) on random
+    data with padding at every edge: one kernel, bit-exact with the FP32 program (the sums
+    keep the fold's order, no contraction). The same translated view taken once per term is
+    gathered once."""
+    from compute import trees
+    from oracle import inputs
+    T = gpu_tensors
+    B, H, W, D, F = 2, 9, 11, 3, 4
+    x = inputs.normal(B * H * W * D, 61).reshape(B, H, W, D)
+    w = inputs.normal(3 * 3 * D * F, 62).reshape(3, 3, D, F)
+    b = inputs.normal(F, 63)
+    tx, tw, tb = T.Tensor(x), T.Tensor(w), T.Tensor(b)
+    out = spec_cases.convolute(T, tx, tw, tb)
+    clo = trees.Concatenate([p.closure for p in out._parts])
+    got = np.asarray(out.flatArray(), np.float32)
+    data = {id(tx): x, id(tw): w, id(tb): b}
+    leaves = [data[id(p.id)] for p in fp64._param_order(clo)]  # the kernel's first-visit order
+    want = fp64.evaluate(clo, (B, H, W), leaves, "fp32").ravel()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    src = T.kernel_source(clo, (B, H, W))
+    assert "nodes, 3 leaves" in src
