@@ -86,6 +86,37 @@ def test_rows_and_cols_ragged(gpu_tensors, shape):
     assert np.all(np.abs(cs - c64) <= 4 * fp64.EPS32 * np.abs(a).astype(np.float64).sum(axis=0) + 1e-30)
 
 
+def test_total_of_many_rows(gpu_tensors):
+    """The FP64 combine over one partial per row: row counts that fill the eight loads in
+    flight per thread exactly, leave an odd partial, and take several rounds of loads (131071
+    and 300001 rows). Each total is computed twice, each time after a small sum on the same
+    stream, and must give the same bits both times."""
+    T = gpu_tensors
+    cols = 4
+    small = upload(T, np.ones(10, np.float32))
+    small_out = alloc(T, 1)
+    for rows in (2048, 2049, 131071, 300001):
+        a = inputs.heavy_tails(inputs.normal(rows * cols, 401), 402).reshape(rows, cols)
+        b = upload(T, a)
+        tot, tot64 = alloc(T, 2), alloc(T, 2, np.float64)
+        for k in range(2):
+            capi.check(capi.lib.cs_reduce_sum(T.stream, small.ptr, 10, small_out.ptr, None))
+            capi.check(capi.lib.cs_reduce_rows(T.stream, b.ptr, rows, cols, None, None, tot.ptr + 4 * k,
+                                               tot64.ptr + 8 * k))
+        _, _, sa, total = oracle.row_stats64(a)
+        t = download(T, tot, 2)
+        t64 = download(T, tot64, 2, np.float64)
+        print("rows %d: total %.17g, oracle %.17g" % (rows, t64[0], total), flush=True)
+        assert t64[0] == t64[1] and t[0] == t[1]
+        assert abs(float(t64[0]) - total) <= 1e-12 * sa.sum()
+        assert abs(float(t[0]) - total) <= 4 * fp64.EPS32 * sa.sum()
+        assert download(T, small_out, 1)[0] == 10.0
+        for h in (b, tot, tot64):
+            h.release()
+    for h in (small, small_out):
+        h.release()
+
+
 @pytest.mark.parametrize("kind", ["sum", "max", "rows"])
 def test_combine_is_deterministic(gpu_tensors, kind):
     """Block partials are combined in FP64 in a fixed order: 300 back-to-back launches (each
