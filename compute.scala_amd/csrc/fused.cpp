@@ -1060,7 +1060,7 @@ static cs_status compile_fused(cs_device *device, const cs_node *nodes, int nnod
     auto mod = std::make_shared<ModuleRef>();
     mod->ordinal = device->ordinal;
     mod->module = build_module(device, src, math_options(o.math_mode), &log);
-    auto *f = new cs_fused();
+    std::unique_ptr<cs_fused> f(new cs_fused());
     f->device = device;
     f->key = key;
     f->hash = fnv1a(key);
@@ -1081,14 +1081,13 @@ static cs_status compile_fused(cs_device *device, const cs_node *nodes, int nnod
     std::lock_guard<std::recursive_mutex> l(device->mu);
     auto it = device->fused_cache.find(key);
     if (it != device->fused_cache.end()) {  // another thread won the race
-      delete f;
       it->second->refs++;
       *fused = it->second;
       return;
     }
     f->refs = 2;  // one for the cache, one for the caller
-    device->fused_cache.emplace(key, f);
-    *fused = f;
+    device->fused_cache.emplace(key, f.get());
+    *fused = f.release();
   });
 }
 
@@ -1215,10 +1214,30 @@ cs_status cs_fused_sum(cs_stream *stream, cs_fused *fused, const void *const *le
   });
 }
 
+// Takes one reference per entry of out[0..n); on failure drops those taken so far and leaves
+// every entry null.
+static void compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out);
+
 cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out) {
   return guard([&] {
     if (!device) fail(CS_INVALID_DEVICE, "device is null");
     if (n < 0 || (n > 0 && (!descs || !out))) fail(CS_INVALID_VALUE, "null argument");
+    for (int i = 0; i < n; ++i) out[i] = nullptr;
+    try {
+      compile_many(device, descs, n, out);
+    } catch (...) {
+      std::lock_guard<std::recursive_mutex> l(device->mu);
+      for (int i = 0; i < n; ++i) {
+        if (out[i] && --out[i]->refs == 0) delete out[i];
+        out[i] = nullptr;
+      }
+      throw;
+    }
+  });
+}
+
+static void compile_many(cs_device *device, const cs_fused_desc *descs, int n, cs_fused **out) {
+  {
     std::vector<Dag> dags(static_cast<size_t>(n));
     std::vector<Options> opts(static_cast<size_t>(n));
     std::vector<std::string> keys(static_cast<size_t>(n));
@@ -1226,7 +1245,6 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
       const cs_fused_desc &d = descs[i];
       dags[i] = prepare(d.nodes, d.nnodes, d.outputs, d.noutputs, d.out_rank, d.out_shape, &d.options, &opts[i],
                         &keys[i]);
-      out[i] = nullptr;
     }
     // Cached expressions come from the cache; the rest compile together, one module per
     // math mode (the modes differ in compiler options).
@@ -1259,7 +1277,7 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
       mod->module = build_module(device, src.str(), math_options(group.first), &log);
       for (size_t k = 0; k < group.second.size(); ++k) {
         int i = group.second[k];
-        auto *f = new cs_fused();
+        std::unique_ptr<cs_fused> f(new cs_fused());
         f->device = device;
         f->key = keys[i];
         f->hash = fnv1a(keys[i]);
@@ -1282,13 +1300,12 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
         std::lock_guard<std::recursive_mutex> l(device->mu);
         auto it = device->fused_cache.find(keys[i]);
         if (it != device->fused_cache.end()) {
-          delete f;
           it->second->refs++;
           out[i] = it->second;
         } else {
           f->refs = 2;
-          device->fused_cache.emplace(keys[i], f);
-          out[i] = f;
+          device->fused_cache.emplace(keys[i], f.get());
+          out[i] = f.release();
         }
       }
     }
@@ -1301,7 +1318,7 @@ cs_status cs_fused_compile_many(cs_device *device, const cs_fused_desc *descs, i
         it->second->refs++;
         out[i] = it->second;
       }
-  });
+  }
 }
 
 cs_status cs_fused_launch_block(cs_stream *stream, cs_fused *fused, const void *const *leaves, int nleaves,

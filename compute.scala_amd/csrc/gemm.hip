@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restr
                                                             int ld, int rows_p, int kp, int vec,
                                                             float *__restrict__ out) {
   __shared__ float tile[64][65];  // [k][n]
-  const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64, t = threadIdx.x;
+  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64, t = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int k = t / 16 + 16 * i, n = (t % 16) * 4;
@@ -182,7 +182,7 @@ __global__ __launch_bounds__(256) void pack_cols_bf16_kernel(const float *__rest
                                                              int ld, int rows_p, int kp, int vec,
                                                              char *__restrict__ out) {
   __shared__ float tile[64][65];  // [k][n]
-  const int n0 = blockIdx.y * 64, k0 = blockIdx.x * 64, t = threadIdx.x;
+  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64, t = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     const int k = t / 16 + 16 * i, n = (t % 16) * 4;
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__r
                                                                 uint8_t *__restrict__ q,
                                                                 uint8_t *__restrict__ scales) {
   __shared__ float tile[128][65];  // [k][n]; row stride 65 words: lane l of a column read hits bank 4l + j + n
-  const int n0 = blockIdx.y * 64, kb = blockIdx.x, k0 = kb * 128, t = threadIdx.x;
+  const int n0 = blockIdx.x * 64, kb = blockIdx.y, k0 = kb * 128, t = threadIdx.x;
   const int c4 = (t % 16) * 4;  // first of the four columns this thread loads
 #pragma unroll
   for (int i = 0; i < 8; ++i) {
@@ -657,18 +657,18 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
     sy = sx + Mp * (Kp / 128);
     hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) * 16 + 255) / 256)), dim3(256), 0, s,
                        A, M, K, lda, int(Mp), int(Kp), vec_a, reinterpret_cast<uint8_t *>(x), sx);
-    hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Kp / 128), unsigned(Np / 64)), dim3(256), 0, s, B, N,
+    hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Np / 64), unsigned(Kp / 128)), dim3(256), 0, s, B, N,
                        K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<uint8_t *>(y), sy);
   } else if (KIND == kF32) {
     hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
                        lda, int(Mp), int(Kp), vec_a, reinterpret_cast<float *>(x));
-    hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256), 0, s, B,
+    hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned(Np / 64), unsigned((Kp + 63) / 64)), dim3(256), 0, s, B,
                        N, K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<float *>(y));
   } else {
     constexpr bool kSplit = KIND == kBF16X3;
     hipLaunchKernelGGL(pack_rows_bf16_kernel<kSplit>, dim3(unsigned((Mp * (Kp / 8) + 255) / 256)), dim3(256), 0, s,
                        A, M, K, lda, int(Mp), int(Kp), vec_a, x);
-    hipLaunchKernelGGL(pack_cols_bf16_kernel<kSplit>, dim3(unsigned((Kp + 63) / 64), unsigned(Np / 64)), dim3(256),
+    hipLaunchKernelGGL(pack_cols_bf16_kernel<kSplit>, dim3(unsigned(Np / 64), unsigned((Kp + 63) / 64)), dim3(256),
                        0, s, B, N, K, ldb, int(Np), int(Kp), vec_b, y);
   }
   static const bool configured = [] {

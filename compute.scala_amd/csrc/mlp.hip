@@ -161,12 +161,16 @@ cs_status cs_mlp_create(cs_device *device, const int32_t *widths, int nw, uint32
   return guard([&] {
     if (!device || !widths || !mlp) fail(CS_INVALID_VALUE, "null argument");
     if (nw < 2 || nw > 16) fail(CS_INVALID_VALUE, "a network has 2 to 16 widths");
-    auto *net = new cs_mlp();
-    net->device = device;
-    for (int i = 0; i < nw; ++i) {
+    for (int i = 0; i < nw; ++i)
       if (widths[i] < 1 || widths[i] > 4096) fail(CS_INVALID_VALUE, "layer width out of range");
-      net->widths.push_back(widths[i]);
-    }
+    // freed with its device arrays if a later step fails
+    std::unique_ptr<cs_mlp, void (*)(cs_mlp *)> net(new cs_mlp(), [](cs_mlp *x) {
+      for (float *p : {x->params, x->m, x->v})
+        if (p) (void)hipFree(p);
+      delete x;
+    });
+    net->device = device;
+    for (int i = 0; i < nw; ++i) net->widths.push_back(widths[i]);
     std::vector<float> host;
     for (int l = 1; l < nw; ++l) {
       int din = widths[l - 1], dout = widths[l];
@@ -188,7 +192,7 @@ cs_status cs_mlp_create(cs_device *device, const int32_t *widths, int nw, uint32
     CS_HIP(hipMemcpy(net->params, host.data(), host.size() * 4, hipMemcpyHostToDevice));
     CS_HIP(hipMemset(net->m, 0, host.size() * 4));
     CS_HIP(hipMemset(net->v, 0, host.size() * 4));
-    *mlp = net;
+    *mlp = net.release();
   });
 }
 

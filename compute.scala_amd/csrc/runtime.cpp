@@ -460,7 +460,7 @@ cs_status cs_device_open(int ordinal, cs_device **device) {
       fail(CS_DEVICE_NOT_FOUND, "no HIP device with ordinal " + std::to_string(ordinal) +
                                     " (" + std::to_string(n) + " visible)");
     }
-    auto *d = new cs_device();
+    std::unique_ptr<cs_device> d(new cs_device());
     d->ordinal = ordinal;
     CS_HIP(hipSetDevice(ordinal));
     CS_HIP(hipGetDeviceProperties(&d->props, ordinal));
@@ -468,8 +468,8 @@ cs_status cs_device_open(int ordinal, cs_device **device) {
     d->arch = arch.substr(0, arch.find(':'));
     d->arena = std::make_unique<Arena>(ordinal);
     d->open_count = 1;
-    g_devices[ordinal] = d;
-    *device = d;
+    g_devices[ordinal] = d.get();
+    *device = d.release();
   });
 }
 
@@ -577,10 +577,16 @@ cs_status cs_event_record(cs_stream *stream, cs_event **event) {
   return guard([&] {
     if (!stream || !event) fail(CS_INVALID_VALUE, "null argument");
     set_device(stream->device);
+    hipEvent_t h = nullptr;
+    CS_HIP(hipEventCreate(&h));
+    hipError_t r = hipEventRecord(h, stream->stream);
+    if (r != hipSuccess) {
+      (void)hipEventDestroy(h);
+      CS_HIP(r);
+    }
     auto *e = new cs_event();
     e->ordinal = stream->device->ordinal;
-    CS_HIP(hipEventCreate(&e->event));
-    CS_HIP(hipEventRecord(e->event, stream->stream));
+    e->event = h;
     *event = e;
   });
 }
@@ -665,11 +671,11 @@ cs_status cs_buffer_alloc(cs_device *device, size_t bytes, cs_buffer **buffer) {
                                        " bytes exceeds device memory");
     set_device(device);
     std::lock_guard<std::recursive_mutex> l(device->mu);
-    auto *b = new cs_buffer();
+    std::unique_ptr<cs_buffer> b(new cs_buffer());
     b->device = device;
     b->block = device->arena->allocate(bytes);
     b->bytes = bytes;
-    *buffer = b;
+    *buffer = b.release();
   });
 }
 
@@ -939,9 +945,17 @@ cs_status cs_graph_begin_capture(cs_stream *stream) {
 cs_status cs_graph_end_capture(cs_stream *stream, cs_graph **graph) {
   return guard([&] {
     if (!stream || !graph) fail(CS_INVALID_VALUE, "null argument");
+    hipGraph_t h = nullptr;
+    CS_HIP(hipStreamEndCapture(stream->stream, &h));
+    hipGraphExec_t x = nullptr;
+    hipError_t r = hipGraphInstantiate(&x, h, nullptr, nullptr, 0);
+    if (r != hipSuccess) {
+      (void)hipGraphDestroy(h);
+      CS_HIP(r);
+    }
     auto *g = new cs_graph();
-    CS_HIP(hipStreamEndCapture(stream->stream, &g->graph));
-    CS_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    g->graph = h;
+    g->exec = x;
     *graph = g;
   });
 }
