@@ -87,16 +87,38 @@ __global__ __launch_bounds__(256) void pack_rows_f32_kernel(const float *__restr
   *reinterpret_cast<float4 *>(out + int64_t(r) * kp + k) = v;
 }
 
-// B (K x N, N contiguous) -> Y[rows_p][kp], row n = column n of B: 64 K x 64 N tiles through
-// LDS, 16-byte loads along N (`vec`) and 16-byte stores along K.
-__global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restrict__ src, int n_src, int k_src,
-                                                            int ld, int rows_p, int kp, int vec,
-                                                            float *__restrict__ out) {
-  __shared__ float tile[64][65];  // [k][n]
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64, t = threadIdx.x;
+// A TK x TN tile of B (K x N, N contiguous): rows k0.., columns n0.., zero outside the source.
+// TN / 4 threads per row, one 16-byte load each (`vec`), so a row's 4 TN bytes are read
+// contiguously. Column n of row k sits at (n + k) mod TN: the reads of one column down K
+// (lane l at row 4l + j or 8l + e) then spread over the 64 banks without padding.
+constexpr int kPackTN = 64;  // B-tile width of the FP32 and BF16 packs
+template <int TN>
+__device__ __forceinline__ int tile_col(int k, int n) { return (n + k) & (TN - 1); }
+
+template <int TK, int TN, int kThreadsT>
+__device__ __forceinline__ void load_b_tile(float (*tile)[TN], const float *__restrict__ src, int n_src, int k_src,
+                                            int ld, int n0, int k0, int vec) {
+  constexpr int kRowsPerPass = kThreadsT / (TN / 4), kPasses = TK / kRowsPerPass;
+  const int t = threadIdx.x, n = (t % (TN / 4)) * 4;
+  if (vec && n0 + TN <= n_src && k0 + TK <= k_src) {
+    // the tile lies inside the source: all loads in flight before the first LDS write
+    float4 a[kPasses];
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int k = t / 16 + 16 * i, n = (t % 16) * 4;
+    for (int i = 0; i < kPasses; ++i)
+      a[i] = *reinterpret_cast<const float4 *>(src + int64_t(k0 + t / (TN / 4) + kRowsPerPass * i) * ld + (n0 + n));
+#pragma unroll
+    for (int i = 0; i < kPasses; ++i) {
+      const int k = t / (TN / 4) + kRowsPerPass * i;
+      tile[k][tile_col<TN>(k, n)] = a[i].x;
+      tile[k][tile_col<TN>(k, n + 1)] = a[i].y;
+      tile[k][tile_col<TN>(k, n + 2)] = a[i].z;
+      tile[k][tile_col<TN>(k, n + 3)] = a[i].w;
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kPasses; ++i) {
+    const int k = t / (TN / 4) + kRowsPerPass * i;
     const float *p = src + int64_t(k0 + k) * ld + (n0 + n);
     float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     if (k0 + k < k_src) {
@@ -109,17 +131,30 @@ __global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restr
         a.w = n0 + n + 3 < n_src ? p[3] : 0.0f;
       }
     }
-    tile[k][n] = a.x;
-    tile[k][n + 1] = a.y;
-    tile[k][n + 2] = a.z;
-    tile[k][n + 3] = a.w;
+    tile[k][tile_col<TN>(k, n)] = a.x;
+    tile[k][tile_col<TN>(k, n + 1)] = a.y;
+    tile[k][tile_col<TN>(k, n + 2)] = a.z;
+    tile[k][tile_col<TN>(k, n + 3)] = a.w;
   }
+}
+
+// B (K x N, N contiguous) -> Y[rows_p][kp], row n = column n of B: 64 K x TN N tiles through
+// LDS, 16-byte stores along K (each row of Y gets 256 contiguous bytes per tile).
+template <int TN>
+__global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restrict__ src, int n_src, int k_src,
+                                                            int ld, int rows_p, int kp, int vec,
+                                                            float *__restrict__ out) {
+  __shared__ float tile[64][TN];  // [k][rotated n]
+  const int n0 = blockIdx.x * TN, k0 = blockIdx.y * 64, t = threadIdx.x;
+  load_b_tile<64, TN, 256>(tile, src, n_src, k_src, ld, n0, k0, vec);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < TN / 16; ++i) {
     const int n = t / 16 + 16 * i, c = t % 16, k = k0 + 4 * c;
     if (n0 + n >= rows_p || k >= kp) continue;
-    const float4 v = make_float4(tile[4 * c][n], tile[4 * c + 1][n], tile[4 * c + 2][n], tile[4 * c + 3][n]);
+    const float4 v =
+        make_float4(tile[4 * c][tile_col<TN>(4 * c, n)], tile[4 * c + 1][tile_col<TN>(4 * c + 1, n)],
+                    tile[4 * c + 2][tile_col<TN>(4 * c + 2, n)], tile[4 * c + 3][tile_col<TN>(4 * c + 3, n)]);
     *reinterpret_cast<float4 *>(out + int64_t(n0 + n) * kp + k) = v;
   }
 }
@@ -175,42 +210,23 @@ __global__ __launch_bounds__(256) void pack_rows_bf16_kernel(const float *__rest
   store_bf16x8(v, out + int64_t(r) * (int64_t(kp) * (kSplit ? 4 : 2)), k, kSplit);
 }
 
-// B (K x N, N contiguous) -> Y (row n holds column n of B along K): 64 K x 64 N tiles through
-// LDS; loads are 16 bytes along N (`vec`), stores 16 bytes along K.
-template <bool kSplit>
+// B (K x N, N contiguous) -> Y (row n holds column n of B along K): 64 K x TN N tiles
+// through LDS; stores are 16 bytes along K (split: 16 high and 16 low bytes).
+template <bool kSplit, int TN>
 __global__ __launch_bounds__(256) void pack_cols_bf16_kernel(const float *__restrict__ src, int n_src, int k_src,
                                                              int ld, int rows_p, int kp, int vec,
                                                              char *__restrict__ out) {
-  __shared__ float tile[64][65];  // [k][n]
-  const int n0 = blockIdx.x * 64, k0 = blockIdx.y * 64, t = threadIdx.x;
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int k = t / 16 + 16 * i, n = (t % 16) * 4;
-    const float *p = src + int64_t(k0 + k) * ld + (n0 + n);
-    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (k0 + k < k_src) {
-      if (vec && n0 + n + 4 <= n_src) {
-        a = *reinterpret_cast<const float4 *>(p);
-      } else {
-        a.x = n0 + n < n_src ? p[0] : 0.0f;
-        a.y = n0 + n + 1 < n_src ? p[1] : 0.0f;
-        a.z = n0 + n + 2 < n_src ? p[2] : 0.0f;
-        a.w = n0 + n + 3 < n_src ? p[3] : 0.0f;
-      }
-    }
-    tile[k][n] = a.x;
-    tile[k][n + 1] = a.y;
-    tile[k][n + 2] = a.z;
-    tile[k][n + 3] = a.w;
-  }
+  __shared__ float tile[64][TN];  // [k][rotated n]
+  const int n0 = blockIdx.x * TN, k0 = blockIdx.y * 64, t = threadIdx.x;
+  load_b_tile<64, TN, 256>(tile, src, n_src, k_src, ld, n0, k0, vec);
   __syncthreads();
 #pragma unroll
-  for (int i = 0; i < 2; ++i) {
+  for (int i = 0; i < TN / 32; ++i) {
     const int n = t / 8 + 32 * i, c = t % 8, k = k0 + 8 * c;
     if (n0 + n >= rows_p || k >= kp) continue;
     float v[8];
 #pragma unroll
-    for (int e = 0; e < 8; ++e) v[e] = tile[8 * c + e][n];
+    for (int e = 0; e < 8; ++e) v[e] = tile[8 * c + e][tile_col<TN>(8 * c + e, n)];
     store_bf16x8(v, out + int64_t(n0 + n) * (int64_t(kp) * (kSplit ? 4 : 2)), k, kSplit);
   }
 }
@@ -272,8 +288,9 @@ __global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__r
 
 // B (K x N, N contiguous) -> Y[rows_p][kp] e4m3 (row n = column n of B) plus
 // scales[kb * rows_p + n]: a block stages 128 K x 64 N in LDS (16-byte loads along N when
-// `vec`), then 32 lanes quantise one column's 128-wide block, 4 consecutive K values per lane
-// and one 4-byte store each (the column's 128 bytes are contiguous in Y).
+// `vec`, all in flight when the tile lies inside the source), then 32 lanes quantise one
+// column's 128-wide block, 4 consecutive K values per lane and one 4-byte store each (the
+// column's 128 bytes are contiguous in Y).
 __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__restrict__ src, int n_src,
                                                                 int k_src, int ld, int rows_p, int kp, int vec,
                                                                 uint8_t *__restrict__ q,
@@ -281,25 +298,40 @@ __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__r
   __shared__ float tile[128][65];  // [k][n]; row stride 65 words: lane l of a column read hits bank 4l + j + n
   const int n0 = blockIdx.x * 64, kb = blockIdx.y, k0 = kb * 128, t = threadIdx.x;
   const int c4 = (t % 16) * 4;  // first of the four columns this thread loads
+  if (vec && n0 + 64 <= n_src && k0 + 128 <= k_src) {
+    float4 a[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const int k = t / 16 + 16 * i;
-    const float *p = src + int64_t(k0 + k) * ld + (n0 + c4);
-    float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (k0 + k < k_src) {
-      if (vec && n0 + c4 + 4 <= n_src) {
-        a = *reinterpret_cast<const float4 *>(p);
-      } else {
-        a.x = n0 + c4 < n_src ? p[0] : 0.0f;
-        a.y = n0 + c4 + 1 < n_src ? p[1] : 0.0f;
-        a.z = n0 + c4 + 2 < n_src ? p[2] : 0.0f;
-        a.w = n0 + c4 + 3 < n_src ? p[3] : 0.0f;
-      }
+    for (int i = 0; i < 8; ++i)
+      a[i] = *reinterpret_cast<const float4 *>(src + int64_t(k0 + t / 16 + 16 * i) * ld + (n0 + c4));
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = t / 16 + 16 * i;
+      tile[k][c4] = a[i].x;
+      tile[k][c4 + 1] = a[i].y;
+      tile[k][c4 + 2] = a[i].z;
+      tile[k][c4 + 3] = a[i].w;
     }
-    tile[k][c4] = a.x;
-    tile[k][c4 + 1] = a.y;
-    tile[k][c4 + 2] = a.z;
-    tile[k][c4 + 3] = a.w;
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int k = t / 16 + 16 * i;
+      const float *p = src + int64_t(k0 + k) * ld + (n0 + c4);
+      float4 a = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+      if (k0 + k < k_src) {
+        if (vec && n0 + c4 + 4 <= n_src) {
+          a = *reinterpret_cast<const float4 *>(p);
+        } else {
+          a.x = n0 + c4 < n_src ? p[0] : 0.0f;
+          a.y = n0 + c4 + 1 < n_src ? p[1] : 0.0f;
+          a.z = n0 + c4 + 2 < n_src ? p[2] : 0.0f;
+          a.w = n0 + c4 + 3 < n_src ? p[3] : 0.0f;
+        }
+      }
+      tile[k][c4] = a.x;
+      tile[k][c4 + 1] = a.y;
+      tile[k][c4 + 2] = a.z;
+      tile[k][c4 + 3] = a.w;
+    }
   }
   __syncthreads();
   const int l32 = t % 32, half = t / 32;  // 8 half-waves, each on one column at a time
@@ -662,14 +694,16 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
   } else if (KIND == kF32) {
     hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
                        lda, int(Mp), int(Kp), vec_a, reinterpret_cast<float *>(x));
-    hipLaunchKernelGGL(pack_cols_f32_kernel, dim3(unsigned(Np / 64), unsigned((Kp + 63) / 64)), dim3(256), 0, s, B,
-                       N, K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<float *>(y));
+    const dim3 cols_grid(unsigned(Np / kPackTN), unsigned((Kp + 63) / 64));
+    hipLaunchKernelGGL(pack_cols_f32_kernel<kPackTN>, cols_grid, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp),
+                       vec_b, reinterpret_cast<float *>(y));
   } else {
     constexpr bool kSplit = KIND == kBF16X3;
     hipLaunchKernelGGL(pack_rows_bf16_kernel<kSplit>, dim3(unsigned((Mp * (Kp / 8) + 255) / 256)), dim3(256), 0, s,
                        A, M, K, lda, int(Mp), int(Kp), vec_a, x);
-    hipLaunchKernelGGL(pack_cols_bf16_kernel<kSplit>, dim3(unsigned(Np / 64), unsigned((Kp + 63) / 64)), dim3(256),
-                       0, s, B, N, K, ldb, int(Np), int(Kp), vec_b, y);
+    auto cols = pack_cols_bf16_kernel<kSplit, kPackTN>;
+    const dim3 cols_grid(unsigned(Np / kPackTN), unsigned((Kp + 63) / 64));
+    hipLaunchKernelGGL(cols, cols_grid, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), vec_b, y);
   }
   static const bool configured = [] {
     CS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gemm_lds_kernel<KIND>),
