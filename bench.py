@@ -263,7 +263,19 @@ def bench_c3(T, capi, steps, sets=2):
     bytes_per = 4 * n + 4 * rows * 2 + 8
     res = {"rows_gbs": bytes_per * L / (ms * 1e-3) / 1e9, "rows_us": ms * 1e3 / L, "bytes_per_launch": bytes_per}
     lib.cs_graph_release(g)
-    for h in ins + rs + rm + tot:
+    # column sums of the same matrix (an outer-axis reduction: `split(0).reduce(_ + _)`)
+    cs = [T._alloc(cols * 4) for _ in range(sets)]
+    for i in range(sets):  # sizes the stream's scratch before capture
+        capi.check(lib.cs_reduce_cols(T.stream, ins[i].ptr, rows, cols, cs[i].ptr))
+    T.synchronize()
+    g = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_reduce_cols(T.stream, ins[i].ptr, rows, cols,
+                                                                           cs[i].ptr)) for i in range(sets)] * 4)
+    timer.measure(lambda: lib.cs_graph_launch(T.stream, g), 1)
+    ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
+    cbytes = 4 * n + 4 * cols
+    res.update({"cols_gbs": cbytes * L / (ms * 1e-3) / 1e9, "cols_us": ms * 1e3 / L})
+    lib.cs_graph_release(g)
+    for h in ins + rs + rm + tot + cs:
         h.release()
     return res
 

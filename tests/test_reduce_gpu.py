@@ -72,7 +72,8 @@ def test_rows_full_size(gpu_tensors):
         h.release()
 
 
-@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 1023), (129, 4099), (1000, 3), (3, 300001)])
+@pytest.mark.parametrize("shape", [(1, 1), (3, 5), (7, 1023), (129, 4099), (1000, 3), (3, 300001),
+                                   (1, 8), (300, 1024), (1031, 68), (2049, 4)])
 def test_rows_and_cols_ragged(gpu_tensors, shape):
     T = gpu_tensors
     rows, cols = shape
