@@ -13,7 +13,6 @@ import os
 
 import numpy as np
 
-from . import _capi as capi
 from . import batch as batchmod
 from ._capi import check, lib
 

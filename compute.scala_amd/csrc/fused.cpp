@@ -9,6 +9,7 @@
 // a flattened 1-D grid where each thread produces `V` consecutive innermost outputs
 // (`U` times), contiguous leaves are read with one wide load, broadcast leaves with one
 // scalar load, and the outputs are written with wide coalesced stores.
+#include <algorithm>
 #include <cinttypes>
 #include <cmath>
 #include <cstdio>

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(1024) void mse_grad(const float *__restrict__ pred,
   part[threadIdx.x] = acc;
   __syncthreads();
   for (int off = 512; off > 0; off >>= 1) {
-    if (threadIdx.x < off) part[threadIdx.x] += part[threadIdx.x + off];
+    if (int(threadIdx.x) < off) part[threadIdx.x] += part[threadIdx.x + off];
     __syncthreads();
   }
   if (threadIdx.x == 0 && loss) *loss = float(part[0] / double(n));

@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__rest
       for (int k = 0; k < 4; ++k) v[k] = __builtin_nontemporal_load(p4 + i + k * kBlock);
 #pragma unroll
       for (int k = 0; k < 4; ++k) {
-        Acc t;  // pairwise inside the vector, then one compensated add
+        // pairwise inside the vector, then one compensated add
         float s01, e01, s23, e23, s, e;
         two_sum(v[k].x, v[k].y, s01, e01);
         two_sum(v[k].z, v[k].w, s23, e23);

@@ -120,6 +120,7 @@ Block Arena::allocate(size_t bytes) {
   if (try_allocate(bytes, &b)) return b;
   size_t slab_bytes = std::max(kMinSlab, round_up(bytes, kSlabGranule));
   void *p = nullptr;
+  CS_HIP(hipSetDevice(ordinal_));  // the slab belongs to this arena's device, whatever is current
   hipError_t e = hipMalloc(&p, slab_bytes);
   if (e != hipSuccess && slab_bytes > bytes) {
     (void)hipGetLastError();
@@ -149,6 +150,7 @@ void Arena::release(Block b, std::vector<hipEvent_t> events) {
 
 void Arena::trim() {
   drain_pending(true);
+  CS_HIP(hipSetDevice(ordinal_));
   for (auto it = slabs_.begin(); it != slabs_.end();) {
     if ((*it)->in_use == 0) {
       (void)hipFree((*it)->base);

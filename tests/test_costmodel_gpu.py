@@ -1,5 +1,4 @@
 """The cost-model network on the device: gradients against the FP64 oracle, and training."""
-import ctypes
 
 import numpy as np
 import pytest
