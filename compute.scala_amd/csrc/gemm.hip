@@ -288,14 +288,14 @@ __global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__r
 
 // B (K x N, N contiguous) -> Y[rows_p][kp] e4m3 (row n = column n of B) plus
 // scales[kb * rows_p + n]: a block stages 128 K x 64 N in LDS (16-byte loads along N when
-// `vec`, all in flight when the tile lies inside the source), then 32 lanes quantise one
-// column's 128-wide block, 4 consecutive K values per lane and one 4-byte store each (the
+// `vec`, all in flight when the tile lies inside the source), then 4 lanes quantise one
+// column's 128-wide block, 32 consecutive K values per lane and two 16-byte stores each (the
 // column's 128 bytes are contiguous in Y).
 __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__restrict__ src, int n_src,
                                                                 int k_src, int ld, int rows_p, int kp, int vec,
-                                                                uint8_t *__restrict__ q,
+                                                                uint8_t *__restrict__ q_out,
                                                                 uint8_t *__restrict__ scales) {
-  __shared__ float tile[128][65];  // [k][n]; row stride 65 words: lane l of a column read hits bank 4l + j + n
+  __shared__ float tile[128][65];  // [k][n]; row stride 65 words: read j of lane (n, q) hits bank 32q + j + n
   const int n0 = blockIdx.x * 64, kb = blockIdx.y, k0 = kb * 128, t = threadIdx.x;
   const int c4 = (t % 16) * 4;  // first of the four columns this thread loads
   if (vec && n0 + 64 <= n_src && k0 + 128 <= k_src) {
@@ -334,19 +334,28 @@ __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__r
     }
   }
   __syncthreads();
-  const int l32 = t % 32, half = t / 32;  // 8 half-waves, each on one column at a time
-  for (int j = 0; j < 8; ++j) {
-    const int n = half + 8 * j;
-    if (n0 + n >= rows_p) break;
-    const float x0 = tile[4 * l32][n], x1 = tile[4 * l32 + 1][n], x2 = tile[4 * l32 + 2][n],
-                x3 = tile[4 * l32 + 3][n];
-    float m = fmaxf(fmaxf(fabsf(x0), fabsf(x1)), fmaxf(fabsf(x2), fabsf(x3)));
+  // 4 lanes per column, 32 consecutive K values each: lane q of column n quantises K rows
+  // 32q..32q+31 of the tile and stores them as 32 contiguous bytes of Y's row n.
+  const int n = t / 4, q = t % 4;
+  if (n0 + n < rows_p) {  // the 4 lanes of a column agree
+    float x[32];
+    float m = 0.0f;
 #pragma unroll
-    for (int off = 16; off >= 1; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 32));
+    for (int j = 0; j < 32; ++j) {
+      x[j] = tile[32 * q + j][n];
+      m = fmaxf(m, fabsf(x[j]));
+    }
+    m = fmaxf(m, __shfl_xor(m, 1, 4));
+    m = fmaxf(m, __shfl_xor(m, 2, 4));
     const int e = fp8_scale_exponent(m);
     const float inv = ldexpf(1.0f, -e);
-    *reinterpret_cast<uint32_t *>(q + int64_t(n0 + n) * kp + k0 + 4 * l32) = fp8x4(x0, x1, x2, x3, inv);
-    if (l32 == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
+    uint32_t w[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) w[j] = fp8x4(x[4 * j], x[4 * j + 1], x[4 * j + 2], x[4 * j + 3], inv);
+    uint4 *dst = reinterpret_cast<uint4 *>(q_out + int64_t(n0 + n) * kp + k0 + 32 * q);
+    dst[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    if (q == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
   }
 }
 
