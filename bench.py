@@ -174,6 +174,30 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
     lib.cs_graph_release(g)
     lib.cs_graph_release(g_hot)
+    # PCIe-inclusive rate, reported beside `value` and never as it: X and bias handed over as
+    # host arrays through the Tensor API's upload path (pageable memory -> pinned staging ->
+    # device), the kernel, and the output read back to a host array; wall clock per step.
+    hx = np.random.default_rng(7).standard_normal(n).astype(np.float32)
+    hb = np.random.default_rng(8).uniform(-0.5, 0.5, 256).astype(np.float32)
+    hout = np.empty(n, np.float32)
+
+    def host_step():
+        capi.check(lib.cs_buffer_write(T.stream, xs[0].handle, 0, hx.ctypes.data_as(ctypes.c_void_p), 4 * n, 0))
+        capi.check(lib.cs_buffer_write(T.stream, bs[0].handle, 0, hb.ctypes.data_as(ctypes.c_void_p), 4 * 256, 0))
+        launch(0)
+        capi.check(lib.cs_buffer_read(T.stream, outs[0].handle, 0, hout.ctypes.data_as(ctypes.c_void_p), 4 * n))
+
+    host_step()
+    k = 20
+    t0 = time.perf_counter()
+    for _ in range(k):
+        host_step()
+    pcie_us = (time.perf_counter() - t0) * 1e6 / k
+    res["pcie_inclusive"] = {"us_per_step": round(pcie_us, 1), "gbs": round(bytes_per / (pcie_us * 1e-6) / 1e9, 2),
+                             "steps": k, "path": "host arrays in and out: cs_buffer_write (pinned staging), "
+                                                 "the fused kernel, cs_buffer_read; wall clock"}
+    log("c2 with host arrays in and out (PCIe included): %.1f us per step, %.2f GB/s" %
+        (pcie_us, res["pcie_inclusive"]["gbs"]))
     lib.cs_fused_release(f)
     X.release()
     B.release()
@@ -673,7 +697,7 @@ def main():
                                "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
                                "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
-                   "copy_reference": c2["copy_reference"],
+                   "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch, the profiler's per-dispatch figure); `value` is the timed region's
