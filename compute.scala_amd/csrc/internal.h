@@ -102,9 +102,11 @@ class Arena {
 };
 
 // Pinned host staging ring used by host<->device copies.
+// Pinned host slots a transfer moves through chunk by chunk: the host copy of one chunk
+// overlaps the DMA of the previous ones.
 struct Staging {
-  static constexpr size_t kChunk = 16u << 20;  // bytes per slot
-  static constexpr int kSlots = 4;
+  static constexpr size_t kChunk = 4u << 20;  // bytes per slot
+  static constexpr int kSlots = 16;
   char *host = nullptr;                        // kSlots * kChunk pinned bytes
   hipEvent_t done[kSlots] = {};
   bool busy[kSlots] = {};

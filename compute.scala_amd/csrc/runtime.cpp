@@ -761,22 +761,24 @@ static void staged_dtoh(cs_stream *stream, char *dst, const char *src, size_t by
       CS_HIP(hipEventSynchronize(st.done[s]));
       st.busy[s] = false;
     }
-  // Two slots in flight: copy chunk i+1 while the host drains chunk i.
-  size_t chunks = (bytes + Staging::kChunk - 1) / Staging::kChunk;
+  // Every slot in flight: chunk c goes to slot c % kSlots; once the host has drained chunk c,
+  // its slot takes chunk c + kSlots.
+  const size_t chunks = (bytes + Staging::kChunk - 1) / Staging::kChunk;
   auto issue = [&](size_t c) {
-    int slot = static_cast<int>(c % 2);
-    size_t off = c * Staging::kChunk, n = std::min(Staging::kChunk, bytes - off);
+    const int slot = static_cast<int>(c % Staging::kSlots);
+    const size_t off = c * Staging::kChunk, n = std::min(Staging::kChunk, bytes - off);
     CS_HIP(hipMemcpyAsync(st.host + slot * Staging::kChunk, src + off, n, hipMemcpyDeviceToHost,
                           stream->stream));
     CS_HIP(hipEventRecord(st.done[slot], stream->stream));
   };
-  if (chunks) issue(0);
+  size_t issued = 0;
+  for (; issued < chunks && issued < static_cast<size_t>(Staging::kSlots); ++issued) issue(issued);
   for (size_t c = 0; c < chunks; ++c) {
-    if (c + 1 < chunks) issue(c + 1);
-    int slot = static_cast<int>(c % 2);
+    const int slot = static_cast<int>(c % Staging::kSlots);
     CS_HIP(hipEventSynchronize(st.done[slot]));
-    size_t off = c * Staging::kChunk, n = std::min(Staging::kChunk, bytes - off);
+    const size_t off = c * Staging::kChunk, n = std::min(Staging::kChunk, bytes - off);
     std::memcpy(dst + off, st.host + slot * Staging::kChunk, n);
+    if (issued < chunks) issue(issued++);  // slot (c + kSlots) % kSlots == slot
   }
 }
 

@@ -55,6 +55,20 @@ def test_staging_round_trip_large(gpu_tensors):
     b.release()
 
 
+def test_staging_wraps_its_slots(gpu_tensors):
+    """100 MiB and 12,345 bytes: more chunks than pinned slots in both directions, and a short
+    last chunk."""
+    T = gpu_tensors
+    nbytes = (100 << 20) + 12345
+    x = np.frombuffer(np.random.default_rng(1).bytes(nbytes), np.uint8)
+    b = alloc(T, nbytes // 4 + 1)
+    capi.check(lib.cs_buffer_write(T.stream, b.handle, 0, x.ctypes.data_as(ctypes.c_void_p), nbytes, 0))
+    y = np.zeros(nbytes, np.uint8)
+    capi.check(lib.cs_buffer_read(T.stream, b.handle, 0, y.ctypes.data_as(ctypes.c_void_p), nbytes))
+    assert np.array_equal(x, y)
+    b.release()
+
+
 SRC = b"""
 extern "C" __global__ void axpy(float a, const float* x, float* y, int n) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
