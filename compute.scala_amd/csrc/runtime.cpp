@@ -720,6 +720,69 @@ cs_status cs_buffer_size(cs_buffer *buffer, size_t *bytes) {
   });
 }
 
+// Host copies into and out of the pinned slots on two threads: one thread copies at about
+// 33 GB/s, the DMA moves about 56 GB/s. The caller copies the first half of a chunk while a
+// helper thread copies the second. One helper serves every device (callers take turns); it is
+// created on first use in each process (a forked child has no copy of the parent's thread) and
+// never destroyed, so it may outlive static destruction at exit.
+class CopyHelper {
+ public:
+  static CopyHelper &get() {
+    static std::mutex *m = new std::mutex();
+    static CopyHelper *h = nullptr;
+    static pid_t owner = 0;
+    std::lock_guard<std::mutex> l(*m);
+    if (!h || owner != getpid()) {
+      h = new CopyHelper();
+      owner = getpid();
+    }
+    return *h;
+  }
+  void copy(char *dst, const char *src, size_t n) {
+    constexpr size_t kMinSplit = 256u << 10;
+    if (n < kMinSplit) {
+      std::memcpy(dst, src, n);
+      return;
+    }
+    std::lock_guard<std::mutex> turn(call_mu_);
+    const size_t half = (n / 2) & ~size_t(4095);
+    {
+      std::lock_guard<std::mutex> l(mu_);
+      dst_ = dst + half;
+      src_ = src + half;
+      n_ = n - half;
+      pending_ = true;
+    }
+    cv_.notify_one();
+    std::memcpy(dst, src, half);
+    std::unique_lock<std::mutex> l(mu_);
+    done_cv_.wait(l, [&] { return !pending_; });
+  }
+
+ private:
+  CopyHelper() { std::thread([this] { run(); }).detach(); }
+  void run() {
+    std::unique_lock<std::mutex> l(mu_);
+    for (;;) {
+      cv_.wait(l, [&] { return pending_; });
+      char *d = dst_;
+      const char *s = src_;
+      const size_t n = n_;
+      l.unlock();
+      std::memcpy(d, s, n);
+      l.lock();
+      pending_ = false;
+      done_cv_.notify_one();
+    }
+  }
+  std::mutex call_mu_, mu_;
+  std::condition_variable cv_, done_cv_;
+  char *dst_ = nullptr;
+  const char *src_ = nullptr;
+  size_t n_ = 0;
+  bool pending_ = false;
+};
+
 static void staged_htod(cs_stream *stream, char *dst, const char *src, size_t bytes, bool blocking) {
   cs_device *d = stream->device;
   set_device(d);
@@ -737,7 +800,7 @@ static void staged_htod(cs_stream *stream, char *dst, const char *src, size_t by
     if (st.busy[slot]) CS_HIP(hipEventSynchronize(st.done[slot]));
     size_t n = std::min(Staging::kChunk, bytes - done);
     char *h = st.host + static_cast<size_t>(slot) * Staging::kChunk;
-    std::memcpy(h, src + done, n);
+    CopyHelper::get().copy(h, src + done, n);
     CS_HIP(hipMemcpyAsync(dst + done, h, n, hipMemcpyHostToDevice, stream->stream));
     CS_HIP(hipEventRecord(st.done[slot], stream->stream));
     st.busy[slot] = true;
@@ -777,7 +840,7 @@ static void staged_dtoh(cs_stream *stream, char *dst, const char *src, size_t by
     const int slot = static_cast<int>(c % Staging::kSlots);
     CS_HIP(hipEventSynchronize(st.done[slot]));
     const size_t off = c * Staging::kChunk, n = std::min(Staging::kChunk, bytes - off);
-    std::memcpy(dst + off, st.host + slot * Staging::kChunk, n);
+    CopyHelper::get().copy(dst + off, st.host + slot * Staging::kChunk, n);
     if (issued < chunks) issue(issued++);  // slot (c + kSlots) % kSlots == slot
   }
 }
