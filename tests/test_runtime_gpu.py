@@ -55,6 +55,23 @@ def test_staging_round_trip_large(gpu_tensors):
     b.release()
 
 
+def test_non_blocking_write_frees_the_host_memory_at_return(gpu_tensors):
+    """cs_buffer_write with blocking = 0 copies the host bytes into pinned staging before it
+    returns, so the caller may overwrite its array at once: the device keeps what was passed.
+    Sizes below and above the size at which the host copy is split over two threads."""
+    T = gpu_tensors
+    for nbytes in (100 << 10, (9 << 20) + 4):
+        x = np.frombuffer(np.random.default_rng(nbytes).bytes(nbytes), np.uint8).copy()
+        want = x.copy()
+        b = alloc(T, nbytes // 4 + 1)
+        capi.check(lib.cs_buffer_write(T.stream, b.handle, 0, x.ctypes.data_as(ctypes.c_void_p), nbytes, 0))
+        x[:] = 0xA5  # overwritten right after the call returned
+        y = np.zeros(nbytes, np.uint8)
+        capi.check(lib.cs_buffer_read(T.stream, b.handle, 0, y.ctypes.data_as(ctypes.c_void_p), nbytes))
+        assert np.array_equal(y, want)
+        b.release()
+
+
 def test_staging_wraps_its_slots(gpu_tensors):
     """100 MiB and 12,345 bytes: more chunks than pinned slots in both directions, and a short
     last chunk."""
