@@ -68,10 +68,10 @@ __device__ __forceinline__ void tile_coords(int b, int tiles_m, int tiles_n, int
 // FP32 operands for the FP32 kernel, zero padded to [rows_p][kp]. A (K contiguous): one
 // thread per 4 consecutive K values of a row, 16-byte loads when the source is 16-byte
 // aligned (`vec`) and 16-byte stores.
-__global__ __launch_bounds__(256) void pack_rows_f32_kernel(const float *__restrict__ src, int rows, int cols, int ld,
-                                                            int rows_p, int kp, int vec, float *__restrict__ out) {
+__device__ __forceinline__ void pack_rows_f32(unsigned bx, const float *__restrict__ src, int rows, int cols, int ld,
+                                              int rows_p, int kp, int vec, float *__restrict__ out) {
   const int chunks = kp / 4;
-  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t idx = int64_t(bx) * 256 + threadIdx.x;
   if (idx >= int64_t(rows_p) * chunks) return;
   const int r = static_cast<int>(idx / chunks), k = static_cast<int>(idx % chunks) * 4;
   const float *p = src + int64_t(r) * ld + k;
@@ -141,11 +141,11 @@ __device__ __forceinline__ void load_b_tile(float (*tile)[TN], const float *__re
 // B (K x N, N contiguous) -> Y[rows_p][kp], row n = column n of B: 64 K x TN N tiles through
 // LDS, 16-byte stores along K (each row of Y gets 256 contiguous bytes per tile).
 template <int TN>
-__global__ __launch_bounds__(256) void pack_cols_f32_kernel(const float *__restrict__ src, int n_src, int k_src,
-                                                            int ld, int rows_p, int kp, int vec,
-                                                            float *__restrict__ out) {
+__device__ __forceinline__ void pack_cols_f32(unsigned bx, unsigned by, const float *__restrict__ src, int n_src,
+                                              int k_src, int ld, int rows_p, int kp, int vec,
+                                              float *__restrict__ out) {
   __shared__ float tile[64][TN];  // [k][rotated n]
-  const int n0 = blockIdx.x * TN, k0 = blockIdx.y * 64, t = threadIdx.x;
+  const int n0 = int(bx) * TN, k0 = int(by) * 64, t = threadIdx.x;
   load_b_tile<64, TN, 256>(tile, src, n_src, k_src, ld, n0, k0, vec);
   __syncthreads();
 #pragma unroll
@@ -191,11 +191,10 @@ __device__ __forceinline__ void store_bf16x8(const float (&v)[8], char *row, int
 // 16-byte loads when the source is 16-byte aligned (`vec`), and one (split: two) 16-byte
 // stores. Rows and K values beyond the source are zero.
 template <bool kSplit>
-__global__ __launch_bounds__(256) void pack_rows_bf16_kernel(const float *__restrict__ src, int rows, int cols,
-                                                             int ld, int rows_p, int kp, int vec,
-                                                             char *__restrict__ out) {
+__device__ __forceinline__ void pack_rows_bf16(unsigned bx, const float *__restrict__ src, int rows, int cols, int ld,
+                                               int rows_p, int kp, int vec, char *__restrict__ out) {
   const int chunks = kp / 8;
-  const int64_t idx = int64_t(blockIdx.x) * 256 + threadIdx.x;
+  const int64_t idx = int64_t(bx) * 256 + threadIdx.x;
   if (idx >= int64_t(rows_p) * chunks) return;
   const int r = static_cast<int>(idx / chunks), k = static_cast<int>(idx % chunks) * 8;
   float v[8];
@@ -213,11 +212,11 @@ __global__ __launch_bounds__(256) void pack_rows_bf16_kernel(const float *__rest
 // B (K x N, N contiguous) -> Y (row n holds column n of B along K): 64 K x TN N tiles
 // through LDS; stores are 16 bytes along K (split: 16 high and 16 low bytes).
 template <bool kSplit, int TN>
-__global__ __launch_bounds__(256) void pack_cols_bf16_kernel(const float *__restrict__ src, int n_src, int k_src,
-                                                             int ld, int rows_p, int kp, int vec,
-                                                             char *__restrict__ out) {
+__device__ __forceinline__ void pack_cols_bf16(unsigned bx, unsigned by, const float *__restrict__ src, int n_src,
+                                               int k_src, int ld, int rows_p, int kp, int vec,
+                                               char *__restrict__ out) {
   __shared__ float tile[64][TN];  // [k][rotated n]
-  const int n0 = blockIdx.x * TN, k0 = blockIdx.y * 64, t = threadIdx.x;
+  const int n0 = int(bx) * TN, k0 = int(by) * 64, t = threadIdx.x;
   load_b_tile<64, TN, 256>(tile, src, n_src, k_src, ld, n0, k0, vec);
   __syncthreads();
 #pragma unroll
@@ -229,6 +228,39 @@ __global__ __launch_bounds__(256) void pack_cols_bf16_kernel(const float *__rest
     for (int e = 0; e < 8; ++e) v[e] = tile[8 * c + e][tile_col<TN>(8 * c + e, n)];
     store_bf16x8(v, out + int64_t(n0 + n) * (int64_t(kp) * (kSplit ? 4 : 2)), k, kSplit);
   }
+}
+
+// Both operands in one launch: workgroups [0, row_blocks) pack A's rows, the rest B's 64-column
+// tiles (N fastest), so the tail of one operand's pass overlaps the start of the other's and
+// the call pays one dispatch instead of two.
+struct PackArgs {
+  const float *a;
+  int m, k, lda, mp, kp, vec_a;
+  void *x;
+  unsigned row_blocks;
+  const float *b;
+  int n, ldb, np, vec_b;
+  void *y;
+  unsigned cols_gx;
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void pack_operands_kernel(PackArgs p) {
+  const unsigned b = blockIdx.x;
+  if (b < p.row_blocks) {
+    if constexpr (KIND == kF32)
+      pack_rows_f32(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<float *>(p.x));
+    else
+      pack_rows_bf16<KIND == kBF16X3>(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<char *>(p.x));
+    return;
+  }
+  const unsigned c = b - p.row_blocks;
+  if constexpr (KIND == kF32)
+    pack_cols_f32<kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp, p.vec_b,
+                           static_cast<float *>(p.y));
+  else
+    pack_cols_bf16<KIND == kBF16X3, kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp,
+                                              p.vec_b, static_cast<char *>(p.y));
 }
 
 // FP8 quantisation of 128 values: the scale is 2^e with e the smallest integer such that
@@ -693,26 +725,21 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
   hipStream_t s = stream->stream;
   const int vec_a = (lda % 4 == 0 && reinterpret_cast<uintptr_t>(A) % 16 == 0) ? 1 : 0;
   const int vec_b = (ldb % 4 == 0 && reinterpret_cast<uintptr_t>(B) % 16 == 0) ? 1 : 0;
-  if (KIND == kFP8) {
+  if constexpr (KIND == kFP8) {
     sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes);
     sy = sx + Mp * (Kp / 128);
     hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) * 16 + 255) / 256)), dim3(256), 0, s,
                        A, M, K, lda, int(Mp), int(Kp), vec_a, reinterpret_cast<uint8_t *>(x), sx);
     hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Np / 64), unsigned(Kp / 128)), dim3(256), 0, s, B, N,
                        K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<uint8_t *>(y), sy);
-  } else if (KIND == kF32) {
-    hipLaunchKernelGGL(pack_rows_f32_kernel, dim3(unsigned((Mp * (Kp / 4) + 255) / 256)), dim3(256), 0, s, A, M, K,
-                       lda, int(Mp), int(Kp), vec_a, reinterpret_cast<float *>(x));
-    const dim3 cols_grid(unsigned(Np / kPackTN), unsigned((Kp + 63) / 64));
-    hipLaunchKernelGGL(pack_cols_f32_kernel<kPackTN>, cols_grid, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp),
-                       vec_b, reinterpret_cast<float *>(y));
   } else {
-    constexpr bool kSplit = KIND == kBF16X3;
-    hipLaunchKernelGGL(pack_rows_bf16_kernel<kSplit>, dim3(unsigned((Mp * (Kp / 8) + 255) / 256)), dim3(256), 0, s,
-                       A, M, K, lda, int(Mp), int(Kp), vec_a, x);
-    auto cols = pack_cols_bf16_kernel<kSplit, kPackTN>;
-    const dim3 cols_grid(unsigned(Np / kPackTN), unsigned((Kp + 63) / 64));
-    hipLaunchKernelGGL(cols, cols_grid, dim3(256), 0, s, B, N, K, ldb, int(Np), int(Kp), vec_b, y);
+    PackArgs pa;
+    pa.a = A, pa.m = M, pa.k = K, pa.lda = lda, pa.mp = int(Mp), pa.kp = int(Kp), pa.vec_a = vec_a, pa.x = x;
+    pa.row_blocks = unsigned((Mp * (Kp / (KIND == kF32 ? 4 : 8)) + 255) / 256);
+    pa.b = B, pa.n = N, pa.ldb = ldb, pa.np = int(Np), pa.vec_b = vec_b, pa.y = y;
+    pa.cols_gx = unsigned(Np / kPackTN);
+    const int64_t blocks = int64_t(pa.row_blocks) + int64_t(pa.cols_gx) * ((Kp + 63) / 64);
+    hipLaunchKernelGGL(pack_operands_kernel<KIND>, dim3(unsigned(blocks)), dim3(256), 0, s, pa);
   }
   static const bool configured = [] {
     CS_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&gemm_lds_kernel<KIND>),
