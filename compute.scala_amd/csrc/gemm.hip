@@ -230,39 +230,6 @@ __device__ __forceinline__ void pack_cols_bf16(unsigned bx, unsigned by, const f
   }
 }
 
-// Both operands in one launch: workgroups [0, row_blocks) pack A's rows, the rest B's 64-column
-// tiles (N fastest), so the tail of one operand's pass overlaps the start of the other's and
-// the call pays one dispatch instead of two.
-struct PackArgs {
-  const float *a;
-  int m, k, lda, mp, kp, vec_a;
-  void *x;
-  unsigned row_blocks;
-  const float *b;
-  int n, ldb, np, vec_b;
-  void *y;
-  unsigned cols_gx;
-};
-
-template <int KIND>
-__global__ __launch_bounds__(256) void pack_operands_kernel(PackArgs p) {
-  const unsigned b = blockIdx.x;
-  if (b < p.row_blocks) {
-    if constexpr (KIND == kF32)
-      pack_rows_f32(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<float *>(p.x));
-    else
-      pack_rows_bf16<KIND == kBF16X3>(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<char *>(p.x));
-    return;
-  }
-  const unsigned c = b - p.row_blocks;
-  if constexpr (KIND == kF32)
-    pack_cols_f32<kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp, p.vec_b,
-                           static_cast<float *>(p.y));
-  else
-    pack_cols_bf16<KIND == kBF16X3, kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp,
-                                              p.vec_b, static_cast<char *>(p.y));
-}
-
 // FP8 quantisation of 128 values: the scale is 2^e with e the smallest integer such that
 // 448 * 2^e >= amax (|x| / 2^e <= 448, the e4m3 maximum); amax = 0 gives e = 0. Returns the
 // E8M0 scale byte e + 127.
@@ -287,11 +254,10 @@ __device__ __forceinline__ uint32_t fp8x4(float x0, float x1, float x2, float x3
 // A (rows x cols, K contiguous) -> X[rows_p][kp] e4m3 plus scales[kb * rows_p + r]: 16 lanes
 // per 128-wide block of a row, 8 consecutive values per lane (two 16-byte loads when the
 // source is 16-byte aligned, `vec`), one 8-byte store per lane.
-__global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__restrict__ src, int rows, int cols,
-                                                                int ld, int rows_p, int kp, int vec,
-                                                                uint8_t *__restrict__ q,
-                                                                uint8_t *__restrict__ scales) {
-  const int64_t group = (int64_t(blockIdx.x) * 256 + threadIdx.x) / 16;
+__device__ __forceinline__ void quantize_rows_fp8(unsigned bx, const float *__restrict__ src, int rows, int cols,
+                                                  int ld, int rows_p, int kp, int vec, uint8_t *__restrict__ q,
+                                                  uint8_t *__restrict__ scales) {
+  const int64_t group = (int64_t(bx) * 256 + threadIdx.x) / 16;
   const int l16 = threadIdx.x % 16;
   const int blocks = kp / 128;
   if (group >= int64_t(rows_p) * blocks) return;  // whole groups of 16 lanes leave together
@@ -323,12 +289,11 @@ __global__ __launch_bounds__(256) void quantize_rows_fp8_kernel(const float *__r
 // `vec`, all in flight when the tile lies inside the source), then 4 lanes quantise one
 // column's 128-wide block, 32 consecutive K values per lane and two 16-byte stores each (the
 // column's 128 bytes are contiguous in Y).
-__global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__restrict__ src, int n_src,
-                                                                int k_src, int ld, int rows_p, int kp, int vec,
-                                                                uint8_t *__restrict__ q_out,
-                                                                uint8_t *__restrict__ scales) {
+__device__ __forceinline__ void quantize_cols_fp8(unsigned bx, unsigned by, const float *__restrict__ src, int n_src,
+                                                  int k_src, int ld, int rows_p, int kp, int vec,
+                                                  uint8_t *__restrict__ q_out, uint8_t *__restrict__ scales) {
   __shared__ float tile[128][65];  // [k][n]; row stride 65 words: read j of lane (n, q) hits bank 32q + j + n
-  const int n0 = blockIdx.x * 64, kb = blockIdx.y, k0 = kb * 128, t = threadIdx.x;
+  const int n0 = int(bx) * 64, kb = int(by), k0 = kb * 128, t = threadIdx.x;
   const int c4 = (t % 16) * 4;  // first of the four columns this thread loads
   if (vec && n0 + 64 <= n_src && k0 + 128 <= k_src) {
     float4 a[8];
@@ -389,6 +354,45 @@ __global__ __launch_bounds__(256) void quantize_cols_fp8_kernel(const float *__r
     dst[1] = make_uint4(w[4], w[5], w[6], w[7]);
     if (q == 0) scales[int64_t(kb) * rows_p + n0 + n] = static_cast<uint8_t>(e + 127);
   }
+}
+
+// Both operands in one launch: workgroups [0, row_blocks) pack A's rows, the rest B's 64-column
+// tiles (N fastest), so the tail of one operand's pass overlaps the start of the other's and
+// the call pays one dispatch instead of two.
+struct PackArgs {
+  const float *a;
+  int m, k, lda, mp, kp, vec_a;
+  void *x;
+  unsigned row_blocks;
+  const float *b;
+  int n, ldb, np, vec_b;
+  void *y;
+  unsigned cols_gx;
+  uint8_t *sx, *sy;  // FP8: the scales of A's rows and B's columns
+};
+
+template <int KIND>
+__global__ __launch_bounds__(256) void pack_operands_kernel(PackArgs p) {
+  const unsigned b = blockIdx.x;
+  if (b < p.row_blocks) {
+    if constexpr (KIND == kFP8)
+      quantize_rows_fp8(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<uint8_t *>(p.x), p.sx);
+    else if constexpr (KIND == kF32)
+      pack_rows_f32(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<float *>(p.x));
+    else
+      pack_rows_bf16<KIND == kBF16X3>(b, p.a, p.m, p.k, p.lda, p.mp, p.kp, p.vec_a, static_cast<char *>(p.x));
+    return;
+  }
+  const unsigned c = b - p.row_blocks;
+  if constexpr (KIND == kFP8)
+    quantize_cols_fp8(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp, p.vec_b,
+                      static_cast<uint8_t *>(p.y), p.sy);
+  else if constexpr (KIND == kF32)
+    pack_cols_f32<kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp, p.vec_b,
+                           static_cast<float *>(p.y));
+  else
+    pack_cols_bf16<KIND == kBF16X3, kPackTN>(c % p.cols_gx, c / p.cols_gx, p.b, p.n, p.k, p.ldb, p.np, p.kp,
+                                              p.vec_b, static_cast<char *>(p.y));
 }
 
 // ------------------------------------------------------------------------ main kernel
@@ -728,17 +732,23 @@ void run_lds(cs_stream *stream, int M, int N, int K, const float *A, int lda, co
   if constexpr (KIND == kFP8) {
     sx = reinterpret_cast<uint8_t *>(base + xbytes + ybytes);
     sy = sx + Mp * (Kp / 128);
-    hipLaunchKernelGGL(quantize_rows_fp8_kernel, dim3(unsigned((Mp * (Kp / 128) * 16 + 255) / 256)), dim3(256), 0, s,
-                       A, M, K, lda, int(Mp), int(Kp), vec_a, reinterpret_cast<uint8_t *>(x), sx);
-    hipLaunchKernelGGL(quantize_cols_fp8_kernel, dim3(unsigned(Np / 64), unsigned(Kp / 128)), dim3(256), 0, s, B, N,
-                       K, ldb, int(Np), int(Kp), vec_b, reinterpret_cast<uint8_t *>(y), sy);
-  } else {
+  }
+  {
     PackArgs pa;
     pa.a = A, pa.m = M, pa.k = K, pa.lda = lda, pa.mp = int(Mp), pa.kp = int(Kp), pa.vec_a = vec_a, pa.x = x;
-    pa.row_blocks = unsigned((Mp * (Kp / (KIND == kF32 ? 4 : 8)) + 255) / 256);
     pa.b = B, pa.n = N, pa.ldb = ldb, pa.np = int(Np), pa.vec_b = vec_b, pa.y = y;
-    pa.cols_gx = unsigned(Np / kPackTN);
-    const int64_t blocks = int64_t(pa.row_blocks) + int64_t(pa.cols_gx) * ((Kp + 63) / 64);
+    pa.sx = sx, pa.sy = sy;
+    int64_t col_tiles_k;
+    if constexpr (KIND == kFP8) {  // 16 lanes per 128-wide block of a row; 128 K x 64 N column tiles
+      pa.row_blocks = unsigned((Mp * (Kp / 128) * 16 + 255) / 256);
+      pa.cols_gx = unsigned(Np / 64);
+      col_tiles_k = Kp / 128;
+    } else {  // one thread per 4 (FP32) or 8 K values of a row; 64 K x kPackTN N column tiles
+      pa.row_blocks = unsigned((Mp * (Kp / (KIND == kF32 ? 4 : 8)) + 255) / 256);
+      pa.cols_gx = unsigned(Np / kPackTN);
+      col_tiles_k = (Kp + 63) / 64;
+    }
+    const int64_t blocks = int64_t(pa.row_blocks) + int64_t(pa.cols_gx) * col_tiles_k;
     hipLaunchKernelGGL(pack_operands_kernel<KIND>, dim3(unsigned(blocks)), dim3(256), 0, s, pa);
   }
   static const bool configured = [] {
