@@ -1017,7 +1017,7 @@ BatchPlan plan_batch(const std::vector<BatchExpr> &members, int block, int math_
   });
   // Runs of consecutive tiles of about kItemBytes of traffic: few enough items that the
   // single counter is not a bottleneck, many enough to balance the grid.
-  constexpr int64_t kItemBytes = 256 << 10;
+  constexpr int64_t kItemBytes = 128 << 10;
   for (int i : order) {
     int64_t run = std::max<int64_t>(1, kItemBytes / std::max<int64_t>(tile_bytes[i], 1));
     for (int64_t t = 0; t < tiles[i]; t += run) {
