@@ -123,7 +123,7 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
         best_cfg = fixed
         tuning = {"best": fixed, "fixed": True}
     elif tune:
-        cands = autotune.candidates(256, vector_widths=(4, 8), items=(1, 2, 4), blocks=(256, 512),
+        cands = autotune.candidates(256, vector_widths=(4, 8), items=(1, 2, 4), blocks=(128, 256, 512, 1024),
                                     nontemporal=(0, 1))
         results = autotune.tune_fused(T, out.closure, out.shape, leaf_sets, out_ptrs, cands, reps=8)
         best_cfg = results[0][0]
