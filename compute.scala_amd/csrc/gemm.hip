@@ -536,12 +536,14 @@ __global__ __launch_bounds__(kThreads, 1) void gemm_lds_kernel(int M, int N, int
       }
     }
   };
-  // DMAs of the next stage ride in the first 4 MFMA groups of a stage, 4 per group, so they
-  // have most of the stage to land.
+  // DMAs of the next stage ride in the first MFMA groups of a stage so that they have most of
+  // the stage to land: 4 per group in 4 groups, or 8 per group in 2 for FP8, whose MFMAs take
+  // twice the cycles (a group of 4 spans 256 of the stage's 2048).
   auto dma_slot = [&](int group, int ks_next, int buf) {
-    if (group < 4) {
+    constexpr int kPer = KIND == kFP8 ? 8 : 4;
+    if (group < 16 / kPer) {
 #pragma unroll
-      for (int e = 0; e < 4; ++e) dma(4 * group + e, ks_next, buf);
+      for (int e = 0; e < kPer; ++e) dma(kPer * group + e, ks_next, buf);
     }
   };
 
