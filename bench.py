@@ -147,12 +147,32 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
         launch(i)
     T.synchronize()
     g = capture(capi, T.stream, [lambda i=i: launch(i) for i in range(sets)])
+    # exactly `steps` (and `warmup`) launches: whole replays of the 16-set graph, then one
+    # replay of a graph holding the remaining launches
+    tails = {}
+
+    def run_launches(count):
+        for _ in range(count // sets):
+            capi.check(lib.cs_graph_launch(T.stream, g))
+        rem = count % sets
+        if rem:
+            if rem not in tails:
+                tails[rem] = capture(capi, T.stream, [lambda i=i: launch(i) for i in range(rem)])
+            capi.check(lib.cs_graph_launch(T.stream, tails[rem]))
+
     timer = Timer(capi, T.stream)
-    reps_w = max(1, warmup // sets)
-    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps_w)
-    reps = max(1, steps // sets)
-    ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
-    launches = reps * sets
+    steps = max(1, steps)
+    run_launches(steps % sets)  # records the tail graph outside the timed region
+    T.synchronize()
+    if warmup > 0:
+        timer.measure(lambda: run_launches(warmup), 1)
+    ms = timer.measure(lambda: run_launches(steps), 1)
+    launches = steps
+    # the same kernel over a window of at least 4096 launches (about 28 ms), a figure that
+    # does not depend on how short the requested run is
+    long_launches = max(4096, steps)
+    ms_long = timer.measure(lambda: run_launches(long_launches), 1)
+    reps = max(1, long_launches // sets)
     # hot-cache figure: one input set, repeated
     g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
     ms_hot = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_hot)), reps)
@@ -162,18 +182,22 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
     # kernel with no arithmetic reaches at this size.
     g_copy = capture(capi, T.stream, [lambda i=i: capi.check(lib.cs_memcpy_dtod(T.stream, outs[i].ptr, xs[i].ptr, 4 * n))
                                       for i in range(sets)])
-    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_copy)), reps_w)
+    timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_copy)), 2)
     ms_copy = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g_copy)), reps)
     lib.cs_graph_release(g_copy)
     bytes_per = 4 * n + 4 * 256 + 4 * n
     res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
                us_per_launch_events=us_event,
-               copy_reference={"bytes": 8 * n, "us": ms_copy * 1e3 / launches,
-                               "gbs": 8 * n * launches / (ms_copy * 1e-3) / 1e9},
+               long_window={"launches": long_launches, "us_per_launch": ms_long * 1e3 / long_launches,
+                            "gbs": bytes_per * long_launches / (ms_long * 1e-3) / 1e9},
+               copy_reference={"bytes": 8 * n, "us": ms_copy * 1e3 / (reps * sets),
+                               "gbs": 8 * n * reps * sets / (ms_copy * 1e-3) / 1e9},
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
     lib.cs_graph_release(g)
     lib.cs_graph_release(g_hot)
+    for gt in tails.values():
+        lib.cs_graph_release(gt)
     # PCIe-inclusive rate, reported beside `value` and never as it: X and bias handed over as
     # host arrays through the Tensor API's upload path (pageable memory -> pinned staging ->
     # device), the kernel, and the output read back to a host array; wall clock per step.
@@ -312,28 +336,41 @@ def bench_gemm(T, capi, variants, reps=10):
     lib = capi.lib
     n = 4096
     from oracle import fp64, inputs
-    a, b = inputs.c4_inputs(n)
     ba, bb, bc = T._alloc(n * n * 4), T._alloc(n * n * 4), T._alloc(n * n * 4)
-    for buf, arr in ((ba, a), (bb, b)):
-        capi.check(lib.cs_buffer_write(T.stream, buf.handle, 0, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes, 1))
-    rows = np.arange(0, n, 128)
-    c64 = a[rows].astype(np.float64) @ b.astype(np.float64)
+    rows = np.arange(3, n, 16)  # 256 rows of C against FP64 per operand set
+    sets = {}
+    for scaled in (False, True):
+        a, b = inputs.c4_inputs(n, scaled)
+        sets[scaled] = (a, b, a[rows].astype(np.float64) @ b.astype(np.float64))
     timer = Timer(capi, T.stream)
     out = {}
+
+    def upload(a, b):
+        for buf, arr in ((ba, a), (bb, b)):
+            capi.check(lib.cs_buffer_write(T.stream, buf.handle, 0, arr.ctypes.data_as(ctypes.c_void_p), arr.nbytes,
+                                           1))
+
     for name, v in variants:
         def run():
             capi.check(lib.cs_gemm(T.stream, v, n, n, n, ba.ptr, n, bb.ptr, n, bc.ptr, n))
-        run()
-        T.synchronize()
-        c = np.empty((n, n), np.float32)
-        capi.check(lib.cs_memcpy_dtoh(T.stream, c.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(bc.ptr), c.nbytes))
-        err = fp64.gemm_normalized_error(c[rows], a[rows], b, c64)
+        errs = {}
+        for scaled in (True, False):  # the U(-1,1) set last: it is the one timed
+            a, b, c64 = sets[scaled]
+            upload(a, b)
+            run()
+            T.synchronize()
+            c = np.empty((n, n), np.float32)
+            capi.check(lib.cs_memcpy_dtoh(T.stream, c.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(bc.ptr),
+                                          c.nbytes))
+            errs["row_scaled" if scaled else "uniform"] = fp64.gemm_normalized_error(c[rows], a[rows], b, c64)
         timer.measure(run, 2)
         ms = timer.measure(run, reps)
         tflops = 2.0 * n ** 3 * reps / (ms * 1e-3) / 1e12
-        out[name] = {"tflops": tflops, "ms": ms / reps, "normalized_error": err,
+        out[name] = {"tflops": tflops, "ms": ms / reps, "normalized_error": errs["uniform"],
+                     "normalized_error_row_scaled": errs["row_scaled"],
                      "mfma_util": tflops * (3 if name == "bf16x3" else 1) / PEAK_TFLOPS[name]}
-        log("gemm %s: %.1f TFLOP/s, error %.3g" % (name, tflops, err))
+        log("gemm %s: %.1f TFLOP/s, error %.3g (row-scaled set %.3g)" % (name, tflops, errs["uniform"],
+                                                                         errs["row_scaled"]))
     for h in (ba, bb, bc):
         h.release()
     return out
@@ -531,6 +568,20 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
     return res
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
+    name = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline_c2(seconds=10.0):
     """The reference's kernel for the headline expression (refcl-host, precise mode) on all
     host cores, timed on whole launches until about `seconds` have passed."""
@@ -553,6 +604,7 @@ def cpu_baseline_c2(seconds=10.0):
     bytes_per = 4 * X.size + 4 * 256 + 4 * X.size
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": round(bytes_per * runs / el / 1e9, 3), "unit": "GB/s", "cores": threads, "kind": "port",
+            "cpu": cpu_model(),
             "sample": "%d full launches of the workload's reference kernel text (OpenMP NDRange, host C, "
                       "no FMA) in %.1f s" % (runs, el)}
 
@@ -649,7 +701,7 @@ def main():
         (c2["gbs"], c2["ms_per_launch"] * 1e3, c2["us_per_launch_events"], c2["gbs_hot_cache"]))
     also = {}
     if "c1" in only:
-        also["c1_fma_sum"] = bench_c1(T, capi, args.steps)
+        also["c1_fma_sum"] = bench_c1(T, capi, max(2048, args.steps))
         log("c1: %s" % also["c1_fma_sum"])
     if "c3" in only:
         also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16))
@@ -684,6 +736,16 @@ def main():
             t = json.load(f)
         traffic = t["traffic_bytes"]
         traffic_source = "profiles/c2_hbm_traffic.json, launch configuration %s" % (t["config"],)
+    # The profiler's per-dispatch average for the same kernel and launch configuration, from
+    # the committed kernel-trace summary (its session and command are in the file).
+    profiled = None
+    pp = os.path.join(ROOT, "profiles", "c2_kernel_trace.json")
+    if os.path.exists(pp):
+        with open(pp) as f:
+            t = json.load(f)
+        profiled = {"kernel_us": t["average_us"], "achieved": round(c2["bytes_per_launch"] / (t["average_us"] * 1e-6) / 1e9, 2),
+                    "frac": round(c2["bytes_per_launch"] / (t["average_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS, 4),
+                    "source": "profiles/c2_kernel_trace.json (%s)" % t["session"]}
     line = {
         "metric": METRIC,
         "value": round(gbs, 2),
@@ -702,6 +764,7 @@ def main():
                                "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
+                   "long_window": c2["long_window"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch, the profiler's per-dispatch figure); `value` is the timed region's
@@ -709,7 +772,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(kernel_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kernel_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_source, "kernel_us": round(c2["us_per_launch_events"], 3),
-                     "timed_region_us_per_launch": round(c2["ms_per_launch"] * 1e3, 3)},
+                     "timed_region_us_per_launch": round(c2["ms_per_launch"] * 1e3, 3),
+                     "profiled": profiled},
         "cpu_baseline": cpu,
         "also": also,
     }

@@ -154,6 +154,15 @@ Dag canonicalize(const cs_node *nodes, int nnodes, const int32_t *outputs, int n
             fail(CS_INVALID_VALUE, "leaf slots must be numbered in first-visit order: expected " +
                                        std::to_string(next_leaf) + ", got " + std::to_string(n.param));
           if (n.rank < 0 || n.rank > CS_MAX_RANK) fail(CS_INVALID_VALUE, "leaf rank out of range");
+          // Gather offsets are 32-bit sums of index * stride, like the reference's int
+          // indices (and its Array[Float] leaves), so a leaf stays below 2^31 elements.
+          int64_t leaf_elements = 1;
+          for (int d = 0; d < n.rank; ++d) {
+            if (n.shape[d] < 0) fail(CS_INVALID_VALUE, "negative leaf dimension");
+            leaf_elements *= n.shape[d];
+            if (leaf_elements >= (int64_t(1) << 31))
+              fail(CS_INVALID_BUFFER_SIZE, "a leaf is limited to 2^31 - 1 elements");
+          }
           next_leaf++;
         }
         f.stage = 1;
@@ -512,7 +521,8 @@ class Emitter {
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n    if (!live[u]) continue;\n";
     for (int v = 0; v < V; ++v) s << "    csmi_acc(csmi_s, csmi_c, " << value_of(dag_.outputs[0], v) << ");\n";
     s << "  }\n";
-    s << "  double csmi_d = (double)csmi_s + (double)csmi_c;\n";
+    // c only while s is finite: an infinite or NaN sum stays so, and its error terms are NaN
+    s << "  double csmi_d = __builtin_isfinite(csmi_s) ? (double)csmi_s + (double)csmi_c : (double)csmi_s;\n";
     s << "#pragma unroll\n  for (int off = 32; off >= 1; off >>= 1) csmi_d += __shfl_xor(csmi_d, off, 64);\n";
     s << "  __shared__ double csmi_lw[16];\n";
     s << "  if ((threadIdx.x & 63u) == 0u) csmi_lw[threadIdx.x >> 6] = csmi_d;\n";
@@ -879,8 +889,11 @@ class Emitter {
   std::map<std::string, std::string> chunks_;  // aligned source chunks already loaded, by position
 };
 
+// FAST mirrors -cl-unsafe-math-optimizations (OpenCL.scala:1131-1135): reassociation,
+// reciprocals, no signed zeros and contraction, but not finite-math-only, so infinities and
+// NaN keep their meaning (fmaxf/fminf still return the non-NaN operand).
 std::vector<std::string> math_options(int mode) {
-  if (mode == CS_MATH_FAST) return {"-ffast-math", "-ffp-contract=fast"};
+  if (mode == CS_MATH_FAST) return {"-funsafe-math-optimizations", "-ffp-contract=fast"};
   return {"-ffp-contract=off", "-fno-fast-math"};
 }
 

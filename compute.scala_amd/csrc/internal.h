@@ -136,6 +136,9 @@ struct cs_stream {
   // Work on one stream is ordered, so calls on it may share the scratch.
   void *scratch = nullptr;
   size_t scratch_bytes = 0;
+  // Smaller scratch buffers this one replaced. A graph captured on the stream before the
+  // scratch grew still addresses them, so they are freed only with the stream.
+  std::vector<void *> retired_scratch;
 };
 
 struct cs_event {
@@ -173,6 +176,7 @@ struct cs_graph {
 namespace csmi {
 // Stream-owned scratch of at least `bytes`, zero-filled when (re)allocated. Growing it
 // synchronizes the stream, so the first call of a given size must not be under capture.
+// The buffer it replaces stays allocated until the stream is destroyed.
 void *stream_scratch(cs_stream *stream, size_t bytes);
 // FP64 block partials of a sum in the stream's scratch (n of them), and the fixed-order FP64
 // combine that turns n partials into *out (FP32, nullable) and *out64 (nullable).

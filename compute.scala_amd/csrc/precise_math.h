@@ -24,7 +24,7 @@
 #pragma once
 
 __device__ __forceinline__ float csmi_tanhf_p(float x) {
-#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+#pragma clang fp contract(off) reassociate(off)  // the rounding steps are part of the algorithm
   const float a = fabsf(x);
   // |x| >= 0.625: 1 - 2 / (2^t + 1), t = 2a log2(e) >= 1.8; for t >= 128, 2^t is +inf and
   // the reciprocal 0, which gives the 1 that tanh rounds to beyond 9.01
@@ -48,7 +48,7 @@ __device__ __forceinline__ float csmi_tanhf_p(float x) {
 // produces subnormal results; below -104 the result rounds to +0 (as does -inf, for which f
 // would be NaN). No overflow check is needed.
 __device__ __forceinline__ float csmi_expf_np(float x) {
-#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+#pragma clang fp contract(off) reassociate(off)  // the rounding steps are part of the algorithm
   const float n = rintf(x * 1.44269502f);
   // f = x log2(e) - n, |f| <= 1/2 + 2^-17: one rounding for x L1 - n, one for the x L2 term
   const float f = fmaf(x, 1.92596299e-08f, fmaf(x, 1.44269502f, -n));
@@ -57,7 +57,7 @@ __device__ __forceinline__ float csmi_expf_np(float x) {
 }
 
 __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, float &ql) {
-#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+#pragma clang fp contract(off) reassociate(off)  // the rounding steps are part of the algorithm
   n = rintf(x * 1.44269502f);
   const float r1 = fmaf(n, -0.693145751953125f, x);  // n * L1 is exact (L1 has 15 bits)
   const float rh = fmaf(n, -1.42860677e-06f, r1);
@@ -76,7 +76,7 @@ __device__ __forceinline__ void csmi_exp_reduce(float x, float &n, float &qh, fl
 // Branch free: the argument is clamped to [-104, 89] (exp(89) overflows to +inf and
 // exp(-104) rounds to +0, as the unclamped values would), and NaN is selected at the end.
 __device__ __forceinline__ float csmi_expf(float x) {
-#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+#pragma clang fp contract(off) reassociate(off)  // the rounding steps are part of the algorithm
   const float xc = fminf(fmaxf(x, -104.0f), 89.0f);
   float n, qh, ql;
   csmi_exp_reduce(xc, n, qh, ql);
@@ -87,7 +87,7 @@ __device__ __forceinline__ float csmi_expf(float x) {
 }
 
 __device__ __forceinline__ float csmi_tanhf(float x) {
-#pragma clang fp contract(off)  // the rounding steps below are part of the algorithm
+#pragma clang fp contract(off) reassociate(off)  // the rounding steps are part of the algorithm
   const float a = fminf(fabsf(x), 9.5f);  // tanh rounds to 1 beyond 9.01; NaN selected below
   float n, qh, ql;
   csmi_exp_reduce(2.0f * a, n, qh, ql);

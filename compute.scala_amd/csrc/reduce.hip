@@ -36,8 +36,16 @@ __device__ __forceinline__ void two_sum(float a, float b, float &s, float &e) {
   e = (a - (s - bp)) + (b - bp);
 }
 
+// Once the running sum s is infinite or NaN it stays so (inf + finite = inf, inf - inf and
+// NaN + x = NaN), and only then can a TwoSum error term be NaN. So c is used only while s is
+// finite: the result is s itself otherwise, as a plain FP32 sum of the same values gives
+// (inf for [1, inf, 2] or [3e38, 3e38], NaN for [inf, -inf]).
 struct Acc {
   float s = 0.0f, c = 0.0f;
+  __device__ __forceinline__ float result() const { return __builtin_isfinite(s) ? s + c : s; }
+  __device__ __forceinline__ double result64() const {
+    return __builtin_isfinite(s) ? double(s) + double(c) : double(s);
+  }
   __device__ __forceinline__ void add(float x) {
     float t, e;
     two_sum(s, x, t, e);
@@ -186,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restr
   block_reduce(a, m);
   if (threadIdx.x == 0) {
     if (kMax) partialmax[blockIdx.x] = m;
-    else partial64[blockIdx.x] = double(a.s) + double(a.c);
+    else partial64[blockIdx.x] = a.result64();
   }
 }
 
@@ -234,9 +242,9 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__rest
   }
   block_reduce(a, m);
   if (threadIdx.x == 0) {
-    if (row_sum) row_sum[row] = a.s + a.c;
+    if (row_sum) row_sum[row] = a.result();
     if (row_max) row_max[row] = m;
-    if (partial64) partial64[row] = double(a.s) + double(a.c);
+    if (partial64) partial64[row] = a.result64();
   }
 }
 
@@ -262,7 +270,7 @@ __global__ __launch_bounds__(kBlock) void reduce_cols_kernel(const float *__rest
     for (int k = 0; k < 8; ++k) a.add(v[k]);
   }
   for (; r < r1; ++r, p += cols) a.add(__builtin_nontemporal_load(p));
-  partial64[int64_t(blockIdx.y) * cols + col] = double(a.s) + double(a.c);
+  partial64[int64_t(blockIdx.y) * cols + col] = a.result64();
 }
 
 // Four adjacent columns per thread with 16-byte loads, when cols is a multiple of 4 and the
@@ -300,10 +308,10 @@ __global__ __launch_bounds__(kColsBlock) void reduce_cols4_kernel(const float *_
     a3.add(v.w);
   }
   double *o = partial64 + int64_t(blockIdx.y) * cols + c;
-  o[0] = double(a0.s) + double(a0.c);
-  o[1] = double(a1.s) + double(a1.c);
-  o[2] = double(a2.s) + double(a2.c);
-  o[3] = double(a3.s) + double(a3.c);
+  o[0] = a0.result64();
+  o[1] = a1.result64();
+  o[2] = a2.result64();
+  o[3] = a3.result64();
 }
 
 // One thread per column adds its slices in order, eight loads in flight.

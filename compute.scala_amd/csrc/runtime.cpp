@@ -196,7 +196,8 @@ void *stream_scratch(cs_stream *stream, size_t bytes) {
   if (cap != hipStreamCaptureStatusNone)
     fail(CS_INVALID_OPERATION, "stream scratch must grow outside graph capture; run the call once first");
   CS_HIP(hipStreamSynchronize(stream->stream));
-  if (stream->scratch) CS_HIP(hipFree(stream->scratch));
+  // kept, not freed: graphs captured earlier on this stream may still address it
+  if (stream->scratch) stream->retired_scratch.push_back(stream->scratch);
   stream->scratch = nullptr;
   stream->scratch_bytes = 0;
   size_t n = round_up(bytes, 1u << 20);
@@ -549,6 +550,7 @@ cs_status cs_stream_destroy(cs_stream *stream) {
     }
     CS_HIP(hipStreamDestroy(stream->stream));
     if (stream->scratch) CS_HIP(hipFree(stream->scratch));
+    for (void *p : stream->retired_scratch) CS_HIP(hipFree(p));
     delete stream;
   });
 }

@@ -156,17 +156,46 @@ def _ulp32(v):
     return (nxt.astype(np.float64) - v.astype(np.float64))
 
 
-def evaluate_with_bound(closure, shape, leaves, ulps=2.0):
+# Error model of the fast math mode (the counterpart of -cl-unsafe-math-optimizations),
+# measured over every FP32 input by the device checker and stated here as bounds:
+#   exp(x) = 2^(x * log2(e)) with the product rounded once: the rounding of the product
+#            (relative 2^-24, and log2(e) itself rounded) scales with |x|, so the result is
+#            within FAST_EXP_ULPS[0] + FAST_EXP_ULPS[1] * |x| ulp;
+#   log(x) = log2(x) * ln(2) from the hardware base-2 logarithm: within FAST_LOG_ABS
+#            absolute plus FAST_LOG_ULPS ulp (near x = 1, where log is small, the absolute
+#            term dominates);
+#   tanh keeps the precise mode's function (2 ulp), sqrt may use the hardware square root.
+FAST_EXP_ULPS = (2.0, 2.0)
+FAST_LOG_ABS = 2.0 ** -21
+FAST_LOG_ULPS = 2.0
+FAST_SQRT_ULPS = 2.0
+
+
+def evaluate_with_bound(closure, shape, leaves, ulps=2.0, mode="precise"):
     """(value, bound): `value` is the FP32 program with correctly rounded operations
     (mode "fp32" of `evaluate`); `bound` is a first-order bound on how far an
     implementation may land from it when + - * / min max are correctly rounded and each of
     exp, tanh, log and sqrt is within `ulps` ulp of the exact result of its own inputs.
-    Errors propagate through each operation by its derivative (mean value bounds)."""
+    Errors propagate through each operation by its derivative (mean value bounds).
+
+    mode "fast" bounds the fast math mode instead, where the compiler may contract a
+    product into an addition (the product is then not rounded), reassociate sums and
+    divide by multiplying with a reciprocal: every + - * / contributes one ulp of the
+    magnitude it works at (for a chain of additions, the sum of the magnitudes of its
+    terms, which bounds every partial sum of any association), a division two, and exp,
+    log and sqrt follow the FAST_* model above."""
+    fast = mode == "fast"
     shape = tuple(int(s) for s in shape)
     params = _param_order(closure)
     data_of = {id(p): (leaves[i] if isinstance(leaves, (list, tuple)) else leaves[id(p)])
                for i, p in enumerate(params)}
     memo = {}
+    mags = {}  # fast mode: sum of the term magnitudes of an addition chain
+
+    def mag_of(node, v, b):
+        if id(node) in mags:
+            return mags[id(node)]
+        return np.abs(v.astype(np.float64)) + b
 
     def ev(node):
         key = id(node)
@@ -201,7 +230,14 @@ def evaluate_with_bound(closure, shape, leaves, ulps=2.0):
                         deriv = 1.0 / np.maximum(np.abs(a64) - ba, 1e-300)
                     else:
                         deriv = 0.5 / np.sqrt(np.maximum(np.abs(a64) - ba, 1e-300))
-                    out = (v, deriv * ba + (ulps + 0.5) * _ulp32(v))
+                    own = (ulps + 0.5) * _ulp32(v)
+                    if fast and node.op == "exp":
+                        own = (FAST_EXP_ULPS[0] + FAST_EXP_ULPS[1] * (np.abs(a64) + ba) + 0.5) * _ulp32(v)
+                    elif fast and node.op == "log":
+                        own = FAST_LOG_ABS + (FAST_LOG_ULPS + 0.5) * _ulp32(v)
+                    elif fast and node.op == "sqrt":
+                        own = (FAST_SQRT_ULPS + 0.5) * _ulp32(v)
+                    out = (v, deriv * ba + own)
         elif kind == "Binary":
             a, ba = ev(node.lhs)
             b, bb = ev(node.rhs)
@@ -220,6 +256,14 @@ def evaluate_with_bound(closure, shape, leaves, ulps=2.0):
                 else:
                     prop = np.where((ba > 0) | (bb > 0), np.inf, 0.0)
                 rounding = np.where(prop > 0, _ulp32(np.abs(v.astype(np.float64)) + prop), 0.0)
+                if fast and node.op in ("+", "-"):
+                    m = mag_of(node.lhs, a, ba) + mag_of(node.rhs, b, bb)
+                    mags[key] = m
+                    rounding = _ulp32(m + prop)
+                elif fast and node.op == "*":
+                    rounding = _ulp32(np.abs(v.astype(np.float64)) + prop)
+                elif fast and node.op == "/":
+                    rounding = 2.0 * _ulp32(np.abs(v.astype(np.float64)) + prop)
                 if node.op in ("min", "max"):
                     rounding = 0.0
                 out = (v, prop + rounding)

@@ -10,7 +10,7 @@ in for the POCL CPU device the reference targets; the arithmetic is the host com
 and libm's, in one of two modes:
 
   "precise": -ffp-contract=off (no FMA), libm float functions;
-  "fast":    -ffast-math -ffp-contract=fast, the counterpart of the reference's
+  "fast":    -funsafe-math-optimizations -ffp-contract=fast, the counterpart of the reference's
              -cl-unsafe-math-optimizations (OpenCL.scala:1131-1135).
 
 Only tests, smoke checks and the CPU-baseline leg of the benchmark use this module.
@@ -359,7 +359,7 @@ void refcl_run(const int64_t *gs, void **bufs, const int *dim_args) {
 
 MODES = {
     "precise": ["-ffp-contract=off"],
-    "fast": ["-ffast-math", "-ffp-contract=fast"],
+    "fast": ["-funsafe-math-optimizations", "-ffp-contract=fast"],
 }
 
 

@@ -195,3 +195,19 @@ def test_nontemporal_stores_write_through(v):
     src = T.kernel_source((y * y).closure, (3, 7, 8))  # one output: a run of V floats
     assert ("off sc1 nt" in src) == (v in (1, 4))
     compiles(src)
+
+
+def test_leaves_of_2_31_elements_are_rejected(offline_tensors):
+    """Gather offsets are 32-bit (the reference's int indices): slice 2 of split(0) on a
+    [3, 1e9] leaf would address element 2e9 + i, so compiling it fails with
+    CS_INVALID_BUFFER_SIZE instead of emitting an offset that wraps. A leaf just under 2^31
+    elements still compiles."""
+    from compute import tensors
+    T = offline_tensors
+    big = trees.ArrayParameter(object(), 0.0, (3, 1000000000))
+    view = trees.Extract(trees.Transform(big, tensors._split_matrix(2, 0, 2), 1))
+    with pytest.raises(capi.InvalidBufferSize):
+        T.kernel_source(view, (1000000000,))
+    ok = trees.ArrayParameter(object(), 0.0, ((1 << 31) - 1,))
+    src = T.kernel_source(trees.Binary("+", trees.Extract(ok), trees.Extract(ok)), ((1 << 31) - 1,))
+    assert "p0" in src

@@ -51,7 +51,7 @@ def test_gemm_4096(gpu_tensors, variant):
         ba, bb, bc = upload(T, a), upload(T, b), alloc(T, n * n)
         capi.check(capi.lib.cs_gemm(T.stream, VARIANTS[variant], n, n, n, ba.ptr, n, bb.ptr, n, bc.ptr, n))
         c = download(T, bc, n * n).reshape(n, n)
-        rows = np.arange(0, n, 64)
+        rows = np.arange(3, n, 8)  # 512 rows, checked against FP64 in full
         c64 = a[rows].astype(np.float64) @ b.astype(np.float64)
         err = fp64.gemm_normalized_error(c[rows], a[rows], b, c64)
         print("%s 4096 scaled=%s normalized error %.3g" % (variant, scaled, err))

@@ -257,3 +257,57 @@ def test_sum_kernel_is_not_an_elementwise_kernel(gpu_tensors):
         leaf.release()
     finally:
         capi.lib.cs_fused_release(f)
+
+
+def _same_class(got, want):
+    """Infinities with the same sign, NaN where want is NaN; finite values compared by the caller."""
+    got, want = np.asarray(got, np.float32), np.asarray(want, np.float32)
+    return (np.array_equal(np.isnan(got), np.isnan(want)) and
+            np.array_equal(np.isposinf(got), np.isposinf(want)) and
+            np.array_equal(np.isneginf(got), np.isneginf(want)))
+
+
+SPECIALS = [[1.0, np.inf, 2.0], [-np.inf, 1.0], [np.inf, -np.inf], [3e38, 3e38], [3e38, 3e38, -1.0],
+            [np.nan, 1.0], [-3e38, -3e38]]
+
+
+@pytest.mark.parametrize("spread", [False, True])
+@pytest.mark.parametrize("case", range(len(SPECIALS)))
+def test_sums_of_non_finite_values(gpu_tensors, case, spread):
+    """A compensated sum must not turn an infinity or an FP32 overflow into NaN: the full sum,
+    rowSum, colSum, split(d).reduce(_ + _) and the fused sum give what numpy's FP32 sum gives
+    ([1, inf, 2] -> inf, [3e38, 3e38] -> inf, [inf, -inf] -> NaN). With `spread`, the values sit
+    apart among 300,000 zeros, so they reach different threads and workgroups."""
+    import functools
+    T = gpu_tensors
+    vals = np.array(SPECIALS[case], np.float32)
+    n = 300000 if spread else len(vals)
+    x = np.zeros(n, np.float32)
+    where = np.linspace(0, n - 1, len(vals)).astype(np.int64) if spread else np.arange(len(vals))
+    x[where] = vals
+    with np.errstate(over="ignore", invalid="ignore"):
+        want = np.float32(x.sum(dtype=np.float32))
+    t = T.Tensor(x).doCache()
+    zero = T.Tensor(np.zeros(n, np.float32)).doCache()
+    for name, got in (("sum", t.sum().flatArray()[0]), ("fused", (t + zero).sum().flatArray()[0])):
+        assert _same_class(got, want), (name, got, want)
+        if np.isfinite(want):
+            assert abs(float(got) - x.astype(np.float64).sum()) <= 4 * fp64.EPS32 * np.abs(x.astype(np.float64)).sum()
+    # rows: the case in row 1 of three, columns: the case in column 1 of three
+    m = np.zeros((3, n), np.float32)
+    m[1] = x
+    tm = T.Tensor(m).doCache()
+    with np.errstate(over="ignore", invalid="ignore"):
+        want_rows = m.sum(axis=1, dtype=np.float32)
+        want_cols = m.T.copy().sum(axis=0, dtype=np.float32)
+    rs = tm.rowSum().flatArray()
+    assert _same_class(rs, want_rows), (rs, want_rows)
+    if not spread:  # a fold over 300,000 slices is not a program anyone writes
+        fold = functools.reduce(lambda a, b: a + b, list(tm.split(1)))
+        fs = fold.flatArray()
+        assert _same_class(fs, want_rows), (fs, want_rows)
+    tc = T.Tensor(m.T.copy()).doCache()
+    cs = tc.colSum().flatArray()
+    assert _same_class(cs, want_cols), (cs, want_cols)
+    for h in (t, zero, tm, tc):
+        h.release()

@@ -7,6 +7,7 @@ import pytest
 
 from compute import _capi as capi
 from gpu_util import alloc, download, upload
+from oracle import fp64
 
 pytestmark = pytest.mark.gpu
 lib = capi.lib
@@ -195,6 +196,40 @@ def test_graph_replays_fused_and_reduction(gpu_tensors):
     tx.release()
     out.release()
     s.release()
+
+
+def test_graph_replays_after_the_scratch_grows(gpu_tensors):
+    """A sum captured on a fresh stream uses that stream's small scratch; a larger sum run
+    afterwards outside capture grows the scratch. The graph still addresses the old buffer,
+    which must stay allocated: every replay gives the eager sum."""
+    T = gpu_tensors
+    s = ctypes.c_void_p()
+    capi.check(lib.cs_stream_create(T.device, ctypes.byref(s)))
+    small_n, big_n = 1 << 12, 1 << 26
+    x = np.random.default_rng(2).standard_normal(big_n).astype(np.float32)
+    xb = upload(T, x)
+    out, out_big = alloc(T, 1), alloc(T, 1)
+    T.synchronize()
+    capi.check(lib.cs_reduce_sum(s, xb.ptr, small_n, out.ptr, None))  # sizes the scratch for small_n
+    capi.check(lib.cs_stream_synchronize(s))
+    capi.check(lib.cs_graph_begin_capture(s))
+    capi.check(lib.cs_reduce_sum(s, xb.ptr, small_n, out.ptr, None))
+    g = ctypes.c_void_p()
+    capi.check(lib.cs_graph_end_capture(s, ctypes.byref(g)))
+    capi.check(lib.cs_reduce_sum(s, xb.ptr, big_n, out_big.ptr, None))  # grows the scratch
+    want = x[:small_n].astype(np.float64).sum()
+    for _ in range(3):
+        capi.check(lib.cs_fill_uniform(s, out.ptr, 1, 5, -9.0, 9.0))
+        capi.check(lib.cs_graph_launch(s, g))
+        capi.check(lib.cs_reduce_sum(s, xb.ptr, big_n, out_big.ptr, None))
+        capi.check(lib.cs_stream_synchronize(s))
+        got = float(download(T, out, 1)[0])
+        assert abs(got - want) <= fp64.sum_bound(x[:small_n]), (got, want)
+    assert abs(float(download(T, out_big, 1)[0]) - x.astype(np.float64).sum()) <= fp64.sum_bound(x)
+    capi.check(lib.cs_graph_release(g))
+    capi.check(lib.cs_stream_destroy(s))
+    for h in (xb, out, out_big):
+        h.release()
 
 
 def test_release_during_capture(gpu_tensors):
