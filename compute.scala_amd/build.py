@@ -28,10 +28,11 @@ COMMON = [
 
 def _embed_headers():
     """Writes the device headers that generated kernels include as C++ string literals."""
-    with open(os.path.join(CSRC, "precise_math.h")) as f:
-        text = f.read()
-    with open(os.path.join(OBJ, "precise_math.inc"), "w") as f:
-        f.write('static const char kPreciseMathSource[] = R"csmi_src(%s)csmi_src";\n' % text)
+    for header, name in (("precise_math.h", "kPreciseMathSource"), ("combine.h", "kCombineSource")):
+        with open(os.path.join(CSRC, header)) as f:
+            text = f.read().replace("#pragma once\n", "")
+        with open(os.path.join(OBJ, header.replace(".h", ".inc")), "w") as f:
+            f.write('static const char %s[] = R"csmi_src(%s)csmi_src";\n' % (name, text))
 
 
 def _compile(src):

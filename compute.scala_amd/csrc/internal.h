@@ -178,10 +178,15 @@ namespace csmi {
 // synchronizes the stream, so the first call of a given size must not be under capture.
 // The buffer it replaces stays allocated until the stream is destroyed.
 void *stream_scratch(cs_stream *stream, size_t bytes);
-// FP64 block partials of a sum in the stream's scratch (n of them), and the fixed-order FP64
-// combine that turns n partials into *out (FP32, nullable) and *out64 (nullable).
-double *sum_partials(cs_stream *stream, size_t n);
-void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, float *out, double *out64);
+// The arrival counter and room for n FP64 block partials of a ticketed sum (combine.h) in
+// the stream's scratch, and the launch that writes an empty sum (0) to *out and *out64
+// (each nullable).
+struct SumScratch {
+  unsigned *ticket;
+  double *partials;
+};
+SumScratch sum_scratch(cs_stream *stream, size_t n);
+void launch_empty_sum(cs_stream *stream, float *out, double *out64);
 // Events recorded now on every live stream of the device (for stream-ordered frees).
 std::vector<hipEvent_t> record_on_all_streams(cs_device *device);
 // Makes `device` current on the calling thread.
