@@ -194,6 +194,7 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
                                "gbs": 8 * n * reps * sets / (ms_copy * 1e-3) / 1e9},
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
+    res["overlapped"] = c2_overlapped(T, capi, f, ptrs, outs, bytes_per, long_launches)
     lib.cs_graph_release(g)
     lib.cs_graph_release(g_hot)
     for gt in tails.values():
@@ -232,6 +233,103 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
     for h in xs + bs + outs:
         h.release()
     return res
+
+
+def bench_hbm(T, capi, launches=1024):
+    """The library's own streaming kernels (cs_hbm_probe: no arithmetic, the generated
+    kernels' load/store forms and layout) under the headline's harness -- rotating buffer
+    sets, graph replay, HIP events -- at the headline's size (16 MiB read + 16 MiB written per
+    launch, 16 sets) and at 512 MiB per launch: what HBM gives a kernel of that size and shape."""
+    lib = capi.lib
+    out = {}
+    names = {capi.PROBE_STORE_PLAIN: "plain", capi.PROBE_STORE_NT: "nt", capi.PROBE_STORE_NT_WRITE_THROUGH: "sc1 nt"}
+    for label, n, sets in (("16MiB", 1 << 22, 16), ("256MiB", 1 << 26, 2)):
+        srcs = [T._alloc(n * 4) for _ in range(sets)]
+        dsts = [T._alloc(n * 4) for _ in range(sets)]
+        for i in range(sets):
+            capi.check(lib.cs_fill_uniform(T.stream, srcs[i].ptr, n, 7000 + i, 0.0, 1.0))
+        T.synchronize()
+        rows = []
+        combos = [(capi.PROBE_READ, capi.PROBE_STORE_PLAIN, u) for u in (1, 4)]
+        combos += [(k, st, u) for k in (capi.PROBE_WRITE, capi.PROBE_COPY) for st in names for u in (1, 2, 4)]
+        timer = Timer(capi, T.stream)
+        for kind, store, items in combos:
+            def one(i, kind=kind, store=store, items=items):
+                capi.check(lib.cs_hbm_probe(T.stream, kind, store, srcs[i].ptr, dsts[i].ptr, n, items))
+            for i in range(sets):
+                one(i)
+            g = capture(capi, T.stream, [lambda i=i: one(i) for i in range(sets)])
+            reps = max(1, launches // sets) if sets > 2 else 16
+            timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), 2)
+            ms = timer.measure(lambda: capi.check(lib.cs_graph_launch(T.stream, g)), reps)
+            lib.cs_graph_release(g)
+            nbytes = 4 * n * (2 if kind == capi.PROBE_COPY else 1)
+            us = ms * 1e3 / (reps * sets)
+            rows.append({"kind": ["read", "write", "copy"][kind], "store": names[store] if kind else "-",
+                         "items_per_thread": items, "bytes": nbytes, "us": round(us, 3),
+                         "gbs": round(nbytes / (us * 1e-6) / 1e9, 1)})
+        for h in srcs + dsts:
+            h.release()
+        best = {k: max((r for r in rows if r["kind"] == k), key=lambda r: r["gbs"]) for k in ("read", "write", "copy")}
+        out[label] = {"best": best, "all": rows}
+        log("hbm probe %s per array: best read %.0f, write %.0f, copy %.0f GB/s" %
+            (label, best["read"]["gbs"], best["write"]["gbs"], best["copy"]["gbs"]))
+    return out
+
+
+def c2_overlapped(T, capi, f, ptrs, outs, bytes_per, launches, nstreams=4):
+    """The same launches with the 16 independent input sets spread over `nstreams` streams
+    (set i on stream i mod nstreams, each stream replaying a graph of its sets), so that one
+    launch's ramp-down overlaps the next one's start; timed from an event on the launch
+    stream that every stream waits for to events that the launch stream waits for. Reported
+    beside `value`, which runs the launches one after another."""
+    lib = capi.lib
+    sets = len(ptrs)
+    streams = []
+    for _ in range(nstreams):
+        st = ctypes.c_void_p()
+        capi.check(lib.cs_stream_create(T.device, ctypes.byref(st)))
+        streams.append(st)
+    graphs = []
+    for k, st in enumerate(streams):
+        mine = [i for i in range(sets) if i % nstreams == k]
+        for i in mine:
+            capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr))
+        capi.check(lib.cs_stream_synchronize(st))
+        capi.check(lib.cs_graph_begin_capture(st))
+        for i in mine:
+            capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr))
+        g = ctypes.c_void_p()
+        capi.check(lib.cs_graph_end_capture(st, ctypes.byref(g)))
+        graphs.append(g)
+    reps = max(1, launches // sets)
+
+    def run(r):
+        e0 = ctypes.c_void_p()
+        capi.check(lib.cs_event_record(T.stream, ctypes.byref(e0)))
+        for st in streams:
+            capi.check(lib.cs_stream_wait_event(st, e0))
+        for _ in range(r):
+            for st, g in zip(streams, graphs):
+                capi.check(lib.cs_graph_launch(st, g))
+        for st in streams:
+            e = ctypes.c_void_p()
+            capi.check(lib.cs_event_record(st, ctypes.byref(e)))
+            capi.check(lib.cs_stream_wait_event(T.stream, e))
+            lib.cs_event_release(e)
+        lib.cs_event_release(e0)
+
+    timer = Timer(capi, T.stream)
+    timer.measure(lambda: run(2), 1)
+    ms = timer.measure(lambda: run(reps), 1)
+    for g in graphs:
+        lib.cs_graph_release(g)
+    for st in streams:
+        capi.check(lib.cs_stream_synchronize(st))
+        lib.cs_stream_destroy(st)
+    n = reps * sets
+    return {"streams": nstreams, "launches": n, "us_per_launch": ms * 1e3 / n,
+            "gbs": bytes_per * n / (ms * 1e-3) / 1e9}
 
 
 def bench_c1(T, capi, steps, sets=16):
@@ -481,24 +579,57 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
         eset.launch(i, f, 256)
     each_out = eset.outputs()
     eset.clear_outputs()
-    # The persistent batch runs every body with V = 2 (or 1 when the innermost dimension is
-    # odd), U = 2 and 512 threads: the batch's occupancy is set by its largest body, and these
-    # bodies stay within 64 VGPRs (8 waves per SIMD), which measured faster than each
-    # expression's own best standalone configuration.
-    bsel = [fused[tuple(batch_vu)][i] for i in range(len(exprs))]
-    t0 = time.time()
-    b = bm.Batch(eset, bsel, block=512)
-    res["batch_compile_s"] = time.time() - t0
-    gb = bm.Graph(T, b.launch)
-    us_batch = gb.time_us(5)
-    batch_out = eset.outputs()
-    res["batch_bit_identical"] = all(np.array_equal(x.view(np.uint32), y.view(np.uint32))
-                                     for x, y in zip(each_out, batch_out))
-    gb.release()
-    res["batch_info"] = b.info()
-    res["batch_config"] = {"vector_width": batch_vu[0], "items_per_thread": batch_vu[1], "block_size": 512,
-                           "nontemporal_store": nontemporal}
-    b.release()
+    # The persistent batch runs in 512-thread workgroups; its bodies' V and U per expression
+    # come from the cost model (block size fixed at the batch's 512), among the bodies whose
+    # V*U <= 4: the batch's occupancy is set by its largest body, and these stay within 64
+    # VGPRs. Beside it: the fixed V = 2, U = 2 bodies, and the per-expression best of the
+    # exhaustive timing database at 512 threads (the autotuner's choice, the judge of the
+    # model's), each as its own batch.
+    light = [c for c in cfgs if c["block_size"] == 512 and c["vector_width"] * c["items_per_thread"] <= 4]
+    choices = {"fixed": [(batch_vu[0], batch_vu[1])] * len(exprs)}
+    model = cm.load(T, model_path)
+    if model is not None:
+        choices["model"] = [(c["vector_width"], c["items_per_thread"]) for c in (model.choose(e, light) for e in exprs)]
+        model.release()
+    db_path = os.path.join(ROOT, "compute.scala_amd", "compute", "data", "c5_timings.json.gz")
+    if n == 512 and seed == 5001 and os.path.exists(db_path):
+        import gzip
+        with gzip.open(db_path, "rt") as f:
+            db = json.load(f)
+        idx = {tuple(sorted(c.items())): k for k, c in enumerate(db["configs"])}
+        t = {(int(i), int(ci)): us for i, ci, us in db["times"]}
+        ex = []
+        for i in range(len(exprs)):
+            best = min(light, key=lambda c: t[(i, idx[tuple(sorted(c.items()))])])
+            ex.append((best["vector_width"], best["items_per_thread"]))
+        choices["exhaustive"] = ex
+    batches = {}
+    for name, ch in choices.items():
+        bsel = [fused[vu][i] for i, vu in enumerate(ch)]
+        t0 = time.time()
+        b = bm.Batch(eset, bsel, block=512)
+        compile_s = time.time() - t0
+        gb = bm.Graph(T, b.launch)
+        us = gb.time_us(5)
+        out = eset.outputs()
+        ident = all(np.array_equal(x.view(np.uint32), y.view(np.uint32)) for x, y in zip(each_out, out))
+        gb.release()
+        batches[name] = {"us": us, "bit_identical": ident, "compile_s": compile_s, "info": b.info()}
+        b.release()
+        eset.clear_outputs()
+        log("c5 batch with %s bodies: %.1f us, identical=%s" % (name, us, ident))
+    res["batch_variants"] = batches
+    use = "model" if "model" in batches else "fixed"
+    us_batch = batches[use]["us"]
+    res["batch_bodies_from"] = use
+    res["batch_bit_identical"] = batches[use]["bit_identical"]
+    res["batch_info"] = batches[use]["info"]
+    res["batch_compile_s"] = batches[use]["compile_s"]
+    if "exhaustive" in batches and "model" in batches:
+        res["batch_model_regret_vs_exhaustive"] = batches["model"]["us"] / batches["exhaustive"]["us"] - 1.0
+    res["batch_config"] = {"block_size": 512, "nontemporal_store": nontemporal,
+                           "bodies": "V, U per expression from the cost model (V*U <= 4)" if use == "model"
+                           else "V=%d, U=%d" % tuple(batch_vu)}
     res["per_expression_default_us"] = us_def
     res["per_expression_us"] = us_each
     res["batched_us"] = us_batch
@@ -565,6 +696,70 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
     for h in xs + outs + [wb, bb]:
         h.release()
     T.close()
+    return res
+
+
+def bench_split(T, capi, n=64, seed=6001, out_dir=None):
+    """Fusion splits of convolution-shaped joins: the exhaustive autotuner times every cut of
+    each of n joins (1, 2, 4 or 8 kernels plus the gathering kernel), the split cost model is
+    trained on the first 3/4 of the joins (by the device MLP kernels) and its picks on the
+    rest are compared with the exhaustive best and with never splitting. Saves the timings and
+    the model beside the package's data when asked (`--split-sweep`)."""
+    from compute import autotune, costmodel as cm, workloads
+    t0 = time.time()
+    joins = workloads.conv_joins(T, n, seed)
+    bufs, ptrs = [], {}
+    for j in joins:
+        for t, d in zip(j.leaves, j.leaf_data):
+            a = np.ascontiguousarray(d, np.float32).ravel()
+            b = T._alloc(a.nbytes)
+            capi.check(capi.lib.cs_buffer_write(T.stream, b.handle, 0, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 1))
+            bufs.append(b)
+            ptrs[id(t)] = b.ptr
+    times = {}
+    for k, j in enumerate(joins):
+        for g in autotune.split_candidates(j.parts):
+            times[(k, g)] = autotune.time_join(T, j.tensor, ptrs, g)
+        if k % 16 == 15:
+            log("split sweep: %d/%d joins timed (%.0f s)" % (k + 1, n, time.time() - t0))
+    for b in bufs:
+        b.release()
+    ntrain = (3 * n) // 4
+    X, y, grp = [], [], []
+    for (k, g), us in times.items():
+        if k < ntrain:
+            X.append(cm.split_features(joins[k].tensor, g))
+            y.append(us)
+            grp.append(k)
+    model = cm.SplitModel(T, len(X[0]), hidden=32, layers=2, seed=5)
+    hist = model.fit(np.stack(X), np.array(y), epochs=400, batch=32, lr=2e-3, groups=np.array(grp))
+    best = chosen = never = 0.0
+    picks = {}
+    for k in range(ntrain, n):
+        j = joins[k]
+        cands = autotune.split_candidates(j.parts)
+        best += min(times[(k, g)] for g in cands)
+        picks[k] = model.choose_groups(j.tensor)
+        chosen += times[(k, picks[k])]
+        never += times[(k, 1)]
+    res = {"joins": n, "seed": seed, "timings": len(times), "held_out": n - ntrain,
+           "train_loss_first_last": [hist[0], hist[-1]],
+           "held_out_us": {"exhaustive": best, "model": chosen, "one_kernel": never},
+           "regret_vs_exhaustive": chosen / best - 1.0, "gain_over_one_kernel": never / chosen,
+           "joins_split_by_exhaustive": sum(1 for k in range(n) if min(autotune.split_candidates(joins[k].parts),
+                                                                     key=lambda g: times[(k, g)]) > 1),
+           "sweep_s": time.time() - t0}
+    log("split model: held-out regret %.3f, gain over one kernel %.3f" % (res["regret_vs_exhaustive"],
+                                                                          res["gain_over_one_kernel"]))
+    if out_dir is not None:
+        import gzip
+        data = os.path.join(ROOT, "compute.scala_amd", "compute", "data")
+        for d in (data, out_dir):
+            os.makedirs(d, exist_ok=True)
+            cm.save(model, os.path.join(d, "splitmodel.npz"))
+            with gzip.open(os.path.join(d, "split_timings.json.gz"), "wt") as f:
+                json.dump({"seed": seed, "joins": n, "times": [[k, g, us] for (k, g), us in sorted(times.items())]}, f)
+    model.release()
     return res
 
 
@@ -661,8 +856,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2048)
     ap.add_argument("--warmup", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="c2,c1,c3,c4,conv,c5")
+    ap.add_argument("--only", default="c2,hbm,c1,c3,c4,conv,c5")
     ap.add_argument("--c5-sweep", action="store_true", help="run the exhaustive autotuner and train the cost model")
+    ap.add_argument("--split-sweep", type=int, default=0,
+                    help="time every fusion split of this many convolution-shaped joins and train the split model")
     ap.add_argument("--out-dir", default=None)
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
@@ -697,9 +894,12 @@ def main():
         v, u, b, nt = (int(x) for x in args.c2_config.split(","))
         fixed = {"vector_width": v, "items_per_thread": u, "block_size": b, "nontemporal_store": nt}
     c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed)
-    log("c2: %.1f GB/s (%.2f us/launch), kernel %.2f us (dispatch events), hot cache %.1f GB/s" %
-        (c2["gbs"], c2["ms_per_launch"] * 1e3, c2["us_per_launch_events"], c2["gbs_hot_cache"]))
+    log("c2: %.1f GB/s (%.2f us/launch), kernel %.2f us (dispatch events), hot cache %.1f GB/s, over %d streams "
+        "%.1f GB/s" % (c2["gbs"], c2["ms_per_launch"] * 1e3, c2["us_per_launch_events"], c2["gbs_hot_cache"],
+                       c2["overlapped"]["streams"], c2["overlapped"]["gbs"]))
     also = {}
+    if "hbm" in only:
+        also["hbm_ceiling"] = bench_hbm(T, capi)
     if "c1" in only:
         also["c1_fma_sum"] = bench_c1(T, capi, max(2048, args.steps))
         log("c1: %s" % also["c1_fma_sum"])
@@ -712,6 +912,9 @@ def main():
         g = also["c4_gemm_4096"]
         if "bf16x3" in g and "fp32" in g:
             g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
+    if args.split_sweep:
+        also["split_sweep"] = bench_split(T, capi, args.split_sweep, out_dir=args.out_dir)
+        log("split sweep: %s" % also["split_sweep"])
     if "conv" in only:
         also["convolution_idiom"] = bench_conv(T, capi)
         log("convolution idiom: %s" % also["convolution_idiom"])
@@ -764,7 +967,7 @@ def main():
                                "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
-                   "long_window": c2["long_window"],
+                   "long_window": c2["long_window"], "overlapped_streams": c2["overlapped"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch, the profiler's per-dispatch figure); `value` is the timed region's
@@ -773,7 +976,13 @@ def main():
                      "frac": round(kernel_gbs / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": traffic_source, "kernel_us": round(c2["us_per_launch_events"], 3),
                      "timed_region_us_per_launch": round(c2["ms_per_launch"] * 1e3, 3),
-                     "profiled": profiled},
+                     "profiled": profiled,
+                     # the same bytes moved by the library's own copy kernel at this size under the
+                     # same harness (no arithmetic): the ceiling for a launch of this size
+                     "copy_ceiling_gbs": (also["hbm_ceiling"]["16MiB"]["best"]["copy"]["gbs"]
+                                          if "hbm_ceiling" in also else None),
+                     "value_over_copy_ceiling": (round(gbs / also["hbm_ceiling"]["16MiB"]["best"]["copy"]["gbs"], 4)
+                                                 if "hbm_ceiling" in also else None)},
         "cpu_baseline": cpu,
         "also": also,
     }

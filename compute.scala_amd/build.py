@@ -17,7 +17,8 @@ LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libcsmi.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip", "mathcheck.hip", "mlp.hip"]
+SOURCES = ["runtime.cpp", "fused.cpp", "reduce.hip", "rng.hip", "gemm.hip", "mathcheck.hip", "mlp.hip",
+           "stream.hip"]
 
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
@@ -28,7 +29,7 @@ COMMON = [
 
 def _embed_headers():
     """Writes the device headers that generated kernels include as C++ string literals."""
-    for header, name in (("precise_math.h", "kPreciseMathSource"), ("combine.h", "kCombineSource")):
+    for header, name in (("precise_math.h", "kPreciseMathSource"),):
         with open(os.path.join(CSRC, header)) as f:
             text = f.read().replace("#pragma once\n", "")
         with open(os.path.join(OBJ, header.replace(".h", ".inc")), "w") as f:

@@ -63,6 +63,8 @@ CHECK_EXP_BOUNDED = 114  # cs_math_check: exp where its argument cannot exceed 8
 OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD, OP_MIN, OP_MAX = 20, 21, 22, 23, 24, 25, 26
 
 MATH_PRECISE, MATH_FAST, MATH_ACCURATE = 0, 1, 2
+PROBE_READ, PROBE_WRITE, PROBE_COPY = 0, 1, 2
+PROBE_STORE_PLAIN, PROBE_STORE_NT, PROBE_STORE_NT_WRITE_THROUGH = 0, 1, 2
 
 GEMM_FP32, GEMM_BF16, GEMM_BF16X3, GEMM_FP8 = 0, 1, 2, 3
 
@@ -170,6 +172,7 @@ _PROTOS = {
     "cs_event_elapsed_ms": (I, [P, P, ctypes.POINTER(F)]),
     "cs_event_on_complete": (I, [P, P, P]),
     "cs_buffer_alloc": (I, [P, SZ, PP]),
+    "cs_buffer_alloc_on": (I, [P, SZ, PP]),
     "cs_buffer_retain": (I, [P]),
     "cs_buffer_release": (I, [P]),
     "cs_buffer_device_ptr": (I, [P, PP]),
@@ -230,6 +233,7 @@ _PROTOS = {
     "cs_fill_normal": (I, [P, P, I64, U32]),
     "cs_gemm": (I, [P, I, I, I, I, P, I, P, I, P, I]),
     "cs_math_check": (I, [P, I, I, U32, U32, P]),
+    "cs_hbm_probe": (I, [P, I, I, P, P, I64, I]),
     "cs_mlp_create": (I, [P, ctypes.POINTER(I32), I, U32, PP]),
     "cs_mlp_param_count": (I, [P, ctypes.POINTER(I64)]),
     "cs_mlp_get_params": (I, [P, P, P, I64]),

@@ -150,3 +150,110 @@ class TimingDB(object):
         with open(tmp, "w") as f:
             json.dump(self.rows, f, indent=0, sort_keys=True)
         os.replace(tmp, self.path)
+
+
+# ------------------------------------------------------------------------ fusion splits
+SPLIT_CONFIG = {"vector_width": 1, "items_per_thread": 1, "block_size": 256, "nontemporal_store": 0}
+
+
+def split_candidates(parts, largest=8):
+    """The group counts a join of `parts` parts can be cut into: divisors up to `largest`."""
+    return [g for g in (1, 2, 4, 8, 16) if g <= largest and parts % g == 0]
+
+
+def time_join(tensors, join, leaf_ptrs, groups, cfg=SPLIT_CONFIG, reps=5):
+    """Microseconds to materialise the join tensor `join` (a Tensor.join) cut into `groups`
+    kernels plus the gathering kernel (groups > 1), replayed from a captured graph of those
+    launches and timed with HIP events. leaf_ptrs maps id(leaf tensor) to its device pointer."""
+    from . import tensors as tz
+    from . import trees
+    heads, per = join.closures(groups)
+    head = tuple(join._head_shape)
+    n_head = 1
+    for s in head:
+        n_head *= s
+    plan, keep = [], []
+    try:
+        if groups == 1:
+            heads = [trees.Concatenate([p.closure for p in join._parts])]
+            per = len(join._parts)
+        part_ptrs = []
+        for h in heads:
+            f, leaves = compile_with(tensors, h, head, cfg)
+            out = tensors._alloc(max(n_head * per, 1) * 4)
+            keep.append(out)
+            plan.append((f, [leaf_ptrs[id(l.id)] for l in leaves], out.ptr))
+            part_ptrs.append(out.ptr)
+        if groups > 1:
+            shells = [tz.CachedTensor(tensors, head + (per,), 0.0, None) for _ in range(groups)]
+            ptr_of = {id(s): p for s, p in zip(shells, part_ptrs)}
+            r = len(head)
+            gather = trees.Concatenate([trees.Extract(trees.Transform(s.arrayTerm, tz._split_matrix(r + 1, r, j), r))
+                                        for s in shells for j in range(per)])
+            f, leaves = compile_with(tensors, gather, head, cfg)
+            out = tensors._alloc(max(n_head * len(join._parts), 1) * 4)
+            keep.append(out)
+            plan.append((f, [ptr_of[id(l.id)] for l in leaves], out.ptr))
+        arrays = [(ctypes.c_void_p * max(len(p), 1))(*p) for _, p, _ in plan]
+
+        def run():
+            for (f, p, o), a in zip(plan, arrays):
+                check(lib.cs_fused_launch(tensors.stream, f, a, len(p), o))
+
+        run()
+        tensors.synchronize()
+        check(lib.cs_graph_begin_capture(tensors.stream))
+        try:
+            run()
+        finally:
+            g = ctypes.c_void_p()
+            check(lib.cs_graph_end_capture(tensors.stream, ctypes.byref(g)))
+        try:
+            check(lib.cs_graph_launch(tensors.stream, g))
+            e0, e1 = ctypes.c_void_p(), ctypes.c_void_p()
+            check(lib.cs_stream_synchronize(tensors.stream))
+            check(lib.cs_event_record(tensors.stream, ctypes.byref(e0)))
+            for _ in range(reps):
+                check(lib.cs_graph_launch(tensors.stream, g))
+            check(lib.cs_event_record(tensors.stream, ctypes.byref(e1)))
+            check(lib.cs_event_wait(e1))
+            ms = ctypes.c_float()
+            check(lib.cs_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+            lib.cs_event_release(e0)
+            lib.cs_event_release(e1)
+        finally:
+            lib.cs_graph_release(g)
+        return ms.value * 1e3 / reps
+    finally:
+        for f, _, _ in plan:
+            lib.cs_fused_release(f)
+        for b in keep:
+            b.release()
+
+
+def split_kernels(join, groups):
+    """(closure, shape) of every kernel a split of `join` into `groups` compiles, the gathering
+    kernel's leaves standing in as shells (for precompiling into the code-object cache)."""
+    from . import tensors as tz
+    from . import trees
+    head = tuple(join._head_shape)
+    if groups == 1:
+        return [(trees.Concatenate([p.closure for p in join._parts]), head)]
+    heads, per = join.closures(groups)
+    shells = [tz.CachedTensor(join._tensors, head + (per,), 0.0, None) for _ in range(groups)]
+    r = len(head)
+    gather = trees.Concatenate([trees.Extract(trees.Transform(s.arrayTerm, tz._split_matrix(r + 1, r, j), r))
+                                for s in shells for j in range(per)])
+    return [(h, head) for h in heads] + [(gather, head)]
+
+
+def precompile(closure, shape, cfg, math_mode=0):
+    """Compiles `closure`'s kernel into the on-disk code-object cache without a device
+    (cs_fused_precompile), so that a later compile of it on a device only loads it."""
+    from . import trees
+    nodes, outputs, _ = trees.serialize(closure)
+    outs = (ctypes.c_int32 * len(outputs))(*outputs)
+    shp = (ctypes.c_int32 * max(len(shape), 1))(*shape)
+    opts = options_struct(math_mode, cfg)
+    check(lib.cs_fused_precompile(nodes, len(nodes), outs, len(outputs), len(shape), shp, ctypes.byref(opts)),
+          "cs_fused_precompile")

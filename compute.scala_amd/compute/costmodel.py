@@ -201,3 +201,67 @@ def measure(eset, fused_by_variant, cfgs, reps=3, min_us=40.0, progress=None):
             out[(i, ci)] = g.time_us(reps) / k
             g.release()
     return out
+
+
+# ------------------------------------------------------------------------ fusion splits
+def _counts(closure):
+    """(nodes, products, sums, leaves, views) of a closure, each shared node counted once."""
+    from . import trees
+    nodes, _, leaves = trees.serialize(closure)
+    ops = [n.op for n in nodes]
+    from ._capi import OP_MUL, OP_ADD, OP_TRANSFORM
+    return (len(nodes), ops.count(OP_MUL), ops.count(OP_ADD), len(leaves), ops.count(OP_TRANSFORM))
+
+
+def split_features(join, groups):
+    """Numeric description of (join, number of kernels it is cut into)."""
+    from . import trees
+    parts = len(join._parts)
+    head = tuple(join._head_shape)
+    n = 1
+    for s in head:
+        n *= s
+    whole = _counts(trees.Concatenate([p.closure for p in join._parts]))
+    heads, per = join.closures(groups)
+    one = _counts(heads[0])
+    f = [
+        math.log2(max(n, 1)), float(len(head)), math.log2(max(head[-1], 1)),
+        float(parts), math.log2(parts), float(groups), math.log2(groups), float(per),
+        math.log2(whole[0]), math.log2(max(whole[1], 1)), float(whole[3]), float(whole[4]),
+        math.log2(one[0]), math.log2(max(one[1], 1)), math.log2(max(one[1], 1) / per),
+        math.log2(max(n, 1) * parts * 4.0), math.log2(max(n, 1)) + math.log2(max(one[1], 1)) + math.log2(groups),
+    ]
+    return np.array(f, dtype=np.float32)
+
+
+class SplitModel(CostModel):
+    """The cost model of fusion splits: predicts the log-time of a join cut into g kernels
+    (relative to the join's mean, as CostModel with groups) and picks g."""
+
+    def choose_groups(self, join, largest=8):
+        from . import autotune
+        cands = autotune.split_candidates(len(join._parts), largest)
+        if len(cands) == 1:
+            return cands[0]
+        X = np.stack([split_features(join, g) for g in cands])
+        return cands[int(np.argmin(self.predict_log_us(X)))]
+
+
+SPLIT_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "splitmodel.npz")
+
+
+def save(model, path):
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    np.savez(path, params=model.net.get_params(), mean=model.mean, std=model.std,
+             ymean=np.array([model.ymean]), widths=np.array(model.net.widths))
+
+
+def load_split_model(tensors, path=SPLIT_MODEL_PATH):
+    """The shipped split model on `tensors`'s device; None when none was saved."""
+    if not os.path.exists(path):
+        return None
+    d = np.load(path)
+    model = SplitModel(tensors, int(d["widths"][0]), hidden=int(d["widths"][1]), layers=len(d["widths"]) - 2)
+    model.net.set_params(d["params"])
+    model.mean, model.std, model.ymean = d["mean"], d["std"], float(d["ymean"][0])
+    return model

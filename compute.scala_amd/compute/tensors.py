@@ -481,9 +481,37 @@ class _JoinTensor(NonInlineTensor):
         self._parts = list(parts)
         self._head_shape = tuple(head.shape)
 
+    def closures(self, groups=1):
+        """The kernels that materialise this join when it is cut into `groups` kernels of
+        consecutive parts (groups divides the number of parts): one Concatenate per group,
+        then, for groups > 1, the kernel that gathers the groups' outputs into the joined
+        layout (its leaves are the group outputs, given as (shape, padding) here)."""
+        per = len(self._parts) // groups
+        heads = [trees.Concatenate([p.closure for p in self._parts[k * per:(k + 1) * per]]) for k in range(groups)]
+        return heads, per
+
     def _materialize(self, memo):
-        closure = trees.Concatenate([p.closure for p in self._parts])
-        return self._tensors._enqueue_closure(closure, self._head_shape, memo)
+        t = self._tensors
+        groups = t._join_groups(self)
+        if groups == 1:
+            closure = trees.Concatenate([p.closure for p in self._parts])
+            return t._enqueue_closure(closure, self._head_shape, memo)
+        # A fusion split: each group of parts is its own kernel writing [head, per]; a last
+        # kernel gathers them into [head, parts]. Only copies are added, so the bits are the
+        # one-kernel join's.
+        heads, per = self.closures(groups)
+        parts = []
+        try:
+            for h in heads:
+                parts.append(CachedTensor(t, self._head_shape + (per,), self.padding,
+                                          t._enqueue_closure(h, self._head_shape, memo)))
+            r = len(self._head_shape)
+            gather = [trees.Extract(trees.Transform(p.arrayTerm, _split_matrix(r + 1, r, j), r))
+                      for p in parts for j in range(per)]
+            return t._enqueue_closure(trees.Concatenate(gather), self._head_shape, memo)
+        finally:
+            for p in parts:
+                p.release()
 
 
 class _ReduceTensor(NonInlineTensor):
@@ -628,6 +656,84 @@ def _split_fold(closure, shape):
     return None
 
 
+def _is_view(node, rank, rows):
+    """Extract(Transform(ArrayParameter, matrix)) of the given output rank: returns
+    (parameter, matrix as rows x (rank + 1) tuples) or None."""
+    if not (isinstance(node, trees.Extract) and isinstance(node.array, trees.Transform) and
+            isinstance(node.array.array, trees.ArrayParameter) and node.array.rank == rank):
+        return None
+    m = node.array.matrix
+    if len(m) != rows * (rank + 1):
+        return None
+    return node.array.array, tuple(tuple(m[r * (rank + 1):(r + 1) * (rank + 1)]) for r in range(rows))
+
+
+def _left_fold_terms(node, op):
+    """The operands of a left fold ((t0 op t1) op t2) ..., in order."""
+    terms = []
+    while isinstance(node, trees.Binary) and node.op == op:
+        terms.append(node.rhs)
+        node = node.lhs
+    terms.append(node)
+    terms.reverse()
+    return terms
+
+
+def _match_mm2(product):
+    """matrixMultiply2 of the reference (README.md:337-346, TensorsSpec.scala:472-479): the
+    tensor split along dimension 1 and folded is A.broadcast([i,j,k]) * B'.broadcast([i,j,k])
+    with A an [i,j] matrix and B' = B.reshape([1,j,k]). Returns (A tensor, B' tensor, i, j, k)
+    or None. Either operand order is accepted (an FP32 product commutes exactly)."""
+    if not isinstance(product, InlineTensor) or len(product.shape) != 3:
+        return None
+    c = product.closure
+    if not (isinstance(c, trees.Binary) and c.op == "*"):
+        return None
+    i, j, k = product.shape
+    for x, y in ((c.lhs, c.rhs), (c.rhs, c.lhs)):
+        va, vb = _is_view(x, 3, 2), _is_view(y, 3, 3)
+        if va is None or vb is None:
+            continue
+        (pa, ma), (pb, mb) = va, vb
+        if (tuple(pa.shape) == (i, j) and ma == ((1, 0, 0, 0), (0, 1, 0, 0)) and
+                tuple(pb.shape) == (1, j, k) and mb == ((0, 0, 0, 0), (0, 1, 0, 0), (0, 0, 1, 0))):
+            return pa.id, pb.id, i, j, k
+    return None
+
+
+def _match_mm1(closure, shape):
+    """matrixMultiply1 of the reference (README.md:312-325, TensorsSpec.scala:506-518):
+    join over the K columns of B of the left fold over j of A.split(1)(j) *
+    B.split(1)(k).split(0)(j).broadcast([i]). Returns (A tensor, B tensor, i, j, k) or None."""
+    if not isinstance(closure, trees.Concatenate) or len(shape) != 1:
+        return None
+    i = shape[0]
+    pa = pb = None
+    J = None
+    for kk, part in enumerate(closure.elements):
+        terms = _left_fold_terms(part, "+")
+        if J is None:
+            J = len(terms)
+        if len(terms) != J:
+            return None
+        for jj, t in enumerate(terms):
+            if not (isinstance(t, trees.Binary) and t.op == "*"):
+                return None
+            l, r = _is_view(t.lhs, 1, 2), _is_view(t.rhs, 1, 2)
+            if l is None or r is None:
+                return None
+            if l[1] != ((1, 0), (0, jj)) or r[1] != ((0, jj), (0, kk)):
+                return None
+            if pa is None:
+                pa, pb = l[0], r[0]
+            if l[0] is not pa or r[0] is not pb:
+                return None
+    K = len(closure.elements)
+    if tuple(pa.shape) != (i, J) or tuple(pb.shape) != (J, K):
+        return None
+    return pa.id, pb.id, i, J, K
+
+
 class TensorSeq(Sequence):
     """The sequence returned by split (Tensors.scala:1038-1073)."""
 
@@ -762,8 +868,21 @@ class Tensors(object):
     Mirror of `trait Tensors` plus the plug-in wiring of `object gpu` (gpu.scala:15-27)."""
 
     def __init__(self, ordinal=0, math_mode=capi.MATH_PRECISE, vector_width=0, items_per_thread=0,
-                 block_size=0, offline=False, axis_reductions=True, fused_sums=True):
+                 block_size=0, offline=False, axis_reductions=True, fused_sums=True, gemm_idioms="large",
+                 join_split=1):
         self.options = capi.CsFusedOptions(math_mode, vector_width, items_per_thread, block_size, 0)
+        # The reference's two matrix-multiply idioms (README.md:301-362) as one FP32 MFMA GEMM:
+        # "large" when the unrolled kernel would hold at least MM1_MIN_TERMS products per
+        # output (matrixMultiply1) or the [i,j,k] product at least MM2_MIN_ELEMENTS elements
+        # (matrixMultiply2); "always"; or "never" (the reference's kernels, in its order)
+        if gemm_idioms not in ("large", "always", "never"):
+            raise IllegalArgumentException("gemm_idioms is one of large, always, never")
+        self.gemm_idioms = gemm_idioms
+        # How many kernels a join is cut into (a fusion split): 1 (one kernel, the reference's
+        # method), a fixed count, or "model" (the learned cost model's choice among the divisors
+        # of the part count up to 8, compute/costmodel.py)
+        self.join_split = join_split
+        self._split_model = None
         # split(d).reduce(+ or max) folds run as one axis reduction (compensated FP32 sums
         # combined in FP64, exact maxima) instead of one kernel with a term per slice
         self.axis_reductions = axis_reductions
@@ -773,6 +892,7 @@ class Tensors(object):
         self.Tensor = TensorCompanion(self)
         self._lock = threading.RLock()
         self.launches = 0
+        self.idiom_gemms = 0  # matrix-multiply idioms run as one GEMM call
         self.offline = offline
         self._device = self._stream = None
         if offline:
@@ -822,8 +942,10 @@ class Tensors(object):
 
     # ------------------------------------------------------------------ internals
     def _alloc(self, nbytes):
+        """A buffer for work on this back-end's stream: stream-ordered, so the host does not
+        wait for the device to finish with blocks released earlier."""
         h = ctypes.c_void_p()
-        check(lib.cs_buffer_alloc(self._device, max(int(nbytes), 1), ctypes.byref(h)), "cs_buffer_alloc")
+        check(lib.cs_buffer_alloc_on(self._stream, max(int(nbytes), 1), ctypes.byref(h)), "cs_buffer_alloc_on")
         return PendingBuffer(self, h.value, int(nbytes))
 
     def _auto_broadcast(self, a, b):
@@ -895,12 +1017,39 @@ class Tensors(object):
                                   ctypes.byref(self.options), ctypes.byref(hit)), "cs_fused_lookup")
         return bool(hit.value)
 
+    def _join_groups(self, join):
+        parts = len(join._parts)
+        if self.join_split == 1 or parts < 2:
+            return 1
+        if self.join_split == "model":
+            from . import costmodel
+            if self._split_model is None:
+                self._split_model = costmodel.load_split_model(self)
+            if self._split_model is None:
+                return 1
+            return self._split_model.choose_groups(join)
+        g = int(self.join_split)
+        if g < 1 or parts % g:
+            raise IllegalArgumentException("join_split %d does not divide %d parts" % (g, parts))
+        return g
+
+    MM1_MIN_TERMS = 1024
+    MM2_MIN_ELEMENTS = 1 << 22
+
     def _enqueue_closure(self, closure, shape, memo):
         """enqueueClosure (Tensors.scala:1291-1392): compile or fetch, materialise the leaves,
         allocate the output, launch."""
-        if self.axis_reductions:
+        if self.gemm_idioms != "never":
+            mm = _match_mm1(closure, shape)
+            if mm is not None and (self.gemm_idioms == "always" or mm[3] * mm[4] >= self.MM1_MIN_TERMS):
+                return self._idiom_gemm(mm, memo)
+        if self.axis_reductions or self.gemm_idioms != "never":
             fold = _split_fold(closure, shape)
-            if fold is not None:
+            if fold is not None and fold[1] == 1 and fold[2] == "+" and self.gemm_idioms != "never":
+                mm = _match_mm2(fold[0].id)
+                if mm is not None and (self.gemm_idioms == "always" or mm[2] * mm[3] * mm[4] >= self.MM2_MIN_ELEMENTS):
+                    return self._idiom_gemm(mm, memo)
+            if fold is not None and self.axis_reductions:
                 out = self._axis_reduce(fold, memo)
                 if out is not None:
                     return out
@@ -919,6 +1068,25 @@ class Tensors(object):
         finally:
             lib.cs_fused_release(f)
 
+
+    def _idiom_gemm(self, mm, memo):
+        """C = A . B for a recognised matrix-multiply idiom: one FP32 MFMA GEMM over the
+        operands' buffers (B given as [j,k] or as its [1,j,k] reshape: the same memory). The
+        [i,j,k] product of matrixMultiply2 is never materialised."""
+        a_t, b_t, i, j, k = mm
+        a = a_t._buffer(memo)
+        try:
+            b = b_t._buffer(memo)
+            try:
+                out = self._alloc(max(i * k, 1) * 4)
+                check(lib.cs_gemm(self._stream, capi.GEMM_FP32, i, k, j, a.ptr, j, b.ptr, k, out.ptr, k), "cs_gemm")
+                self.launches += 1
+                self.idiom_gemms += 1
+                return out
+            finally:
+                b.release()
+        finally:
+            a.release()
 
     def _axis_reduce(self, fold, memo):
         """A recognised split fold over dimension d of a dense [outer, n, inner] buffer: row

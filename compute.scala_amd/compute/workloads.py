@@ -147,3 +147,54 @@ def convolute(T, inp, weight, bias):
                     terms.append(channel.translate([0, dy, dx]) * wt.broadcast((batch, height, width)))
         channels.append(b.broadcast((batch, height, width)) + reduce(lambda x, y: x + y, terms))
     return T.Tensor.join(channels)
+
+
+class JoinExpr(object):
+    """One join of a split workload: its tensor (a Tensor.join), the leaf tensors and their data
+    in argument order, and a description of its shape."""
+
+    def __init__(self, index, tensor, leaves, leaf_data, desc):
+        self.index = index
+        self.tensor = tensor
+        self.leaves = leaves
+        self.leaf_data = leaf_data
+        self.desc = desc
+        self.shape = tuple(tensor.shape)
+        self.parts = len(tensor._parts)
+        self.out_elements = int(np.prod(self.shape))
+        self.leaf_elements = int(sum(d.size for d in leaf_data))
+
+
+def conv_joins(T, n=64, seed=6001):
+    """Joins with many terms per part, the shape of the convolution idiom (convolute): input
+    [batch, height, width, depth], a kh x kw kernel with kh, kw in {1, 3, 5}, 2 to 16 filters
+    (the parts of the join), so a part holds kh*kw*depth products, at most 320 in a join (its
+    one-kernel form compiles in seconds; hiprtc's time grows faster than the body). Sizes are
+    drawn from `seed`; data are U(-1, 1) from the Wang hash."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for e in range(n):
+        depth = int(rng.choice([1, 2, 4, 8]))
+        k = int(rng.choice([1, 3, 5], p=[0.2, 0.6, 0.2]))
+        filters = int(rng.choice([2, 4, 8, 16]))
+        while k * k * depth * filters > 320:
+            if depth > 1:
+                depth //= 2
+            else:
+                filters //= 2
+        pixels = int(round(math.exp(rng.uniform(math.log(2e4), math.log(1e6)))))
+        hw = max(4, int(math.sqrt(pixels / max(1, int(rng.integers(1, 9))))))
+        batch = max(1, pixels // (hw * hw))
+        x = _uniform(batch * hw * hw * depth, seed * 1000 + e * 10).reshape(batch, hw, hw, depth)
+        w = _uniform(k * k * depth * filters, seed * 1000 + e * 10 + 1).reshape(k, k, depth, filters)
+        b = _uniform(filters, seed * 1000 + e * 10 + 2)
+        tx, tw, tb = T.Tensor(x), T.Tensor(w), T.Tensor(b)
+        joined = convolute(T, tx, tw, tb)
+        from . import trees
+        clo = trees.Concatenate([p.closure for p in joined._parts])
+        order = [p.id for p in trees.parameters(clo)]
+        by_id = {id(tx): (tx, x), id(tw): (tw, w), id(tb): (tb, b)}
+        leaves = [by_id[id(t)] for t in order]
+        out.append(JoinExpr(e, joined, [l[0] for l in leaves], [l[1] for l in leaves],
+                            {"batch": batch, "hw": hw, "depth": depth, "kernel": k, "filters": filters}))
+    return out

@@ -20,7 +20,6 @@
 
 #include "internal.h"
 #include "precise_math.inc"  // kPreciseMathSource, generated from precise_math.h
-#include "combine.inc"       // kCombineSource, generated from combine.h
 
 
 namespace csmi {
@@ -415,7 +414,6 @@ class Emitter {
   static std::string prelude(int math_mode) {
     std::string p;
     p += std::string(kPreciseMathSource) + "\n";
-    p += std::string(kCombineSource) + "\n";
     p += "typedef float csmi_f4 __attribute__((ext_vector_type(4), aligned(4)));\n";
     p += "typedef float csmi_f2 __attribute__((ext_vector_type(2), aligned(4)));\n";
     p += "typedef float csmi_v4 __attribute__((ext_vector_type(4)));\n";
@@ -444,7 +442,7 @@ class Emitter {
   void params(std::ostringstream &s) {
     for (int i = 0; i < dag_.nleaves; ++i) s << "const " << gq_ << "float* __restrict__ p" << i << ", ";
     if (o_.sum)
-      s << "double* __restrict__ partials, unsigned* csmi_ticket, float* csmi_out, double* csmi_out64";
+      s << "double* __restrict__ partials";
     else
       s << gq_ << "float* __restrict__ out";
   }
@@ -515,9 +513,8 @@ class Emitter {
 
   // Sum mode, in place of the stores: each thread adds its elements with a compensation term
   // (csmi_acc), the (sum, compensation) pairs become FP64 and are added by a fixed xor-shuffle
-  // tree in each wave and in wave order across the block, and thread 0 publishes the block's
-  // partial; the block that arrives last adds the partials in a fixed order and writes the
-  // sum (combine.h). The element values are the ones the stores would write.
+  // tree in each wave and in wave order across the block, and thread 0 writes the block's
+  // partial. The element values are the ones the stores would write.
   void emit_block_sum(std::ostringstream &s) {
     const int V = o_.V, U = o_.U;
     s << "  float csmi_s = 0.0f, csmi_c = 0.0f;\n";
@@ -530,15 +527,9 @@ class Emitter {
     s << "  __shared__ double csmi_lw[16];\n";
     s << "  if ((threadIdx.x & 63u) == 0u) csmi_lw[threadIdx.x >> 6] = csmi_d;\n";
     s << "  __syncthreads();\n";
-    s << "  __shared__ bool csmi_last;\n";
     s << "  if (threadIdx.x == 0u) {\n    double t = 0.0;\n";
-    s << "    for (unsigned w = 0; w < blockDim.x / 64u; ++w) t += csmi_lw[w];\n";
-    s << "    csmi_last = csmi_publish_f64(partials, blockIdx.x, t, csmi_ticket, gridDim.x);\n  }\n";
-    s << "  __syncthreads();\n";
-    s << "  if (csmi_last) {\n    csmi_acquire_partials();\n";
-    s << "    const double r = csmi_fixed_sum(partials, gridDim.x, (int)blockDim.x);\n";
-    s << "    if (threadIdx.x == 0u) {\n      if (csmi_out) *csmi_out = (float)r;\n"
-         "      if (csmi_out64) *csmi_out64 = r;\n      csmi_reset_ticket(csmi_ticket);\n    }\n  }\n";
+    s << "    for (unsigned w = 0; w < " << bdim_ << " / 64u; ++w) t += csmi_lw[w];\n";
+    s << "    partials[" << (bdim_ == "blockDim.x" ? "blockIdx.x" : "tile") << "] = t;\n  }\n";
   }
 
   // Non-temporal stores. When one store instruction covers the wave's lanes contiguously (a
@@ -1218,25 +1209,21 @@ cs_status cs_fused_sum(cs_stream *stream, cs_fused *fused, const void *const *le
       fail(CS_INVALID_KERNEL_ARGS, "kernel takes " + std::to_string(fused->nleaves) + " leaves, got " +
                                        std::to_string(nleaves));
     set_device(stream->device);
-    if (fused->grid == 0) {
-      launch_empty_sum(stream, out, out64);  // no elements: an empty sum, 0
-      CS_HIP(hipGetLastError());
-      return;
+    double *partials = sum_partials(stream, std::max<size_t>(fused->grid, 1));
+    if (fused->grid > 0) {
+      std::vector<const void *> vals(static_cast<size_t>(nleaves) + 1);
+      std::vector<void *> args(static_cast<size_t>(nleaves) + 1);
+      for (int i = 0; i < nleaves; ++i) {
+        if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
+        vals[i] = leaves[i];
+        args[i] = &vals[i];
+      }
+      vals[nleaves] = partials;
+      args[nleaves] = &vals[nleaves];
+      CS_HIP(hipModuleLaunchKernel(fused->function, fused->grid, 1, 1, fused->block, 1, 1, 0, stream->stream,
+                                   args.data(), nullptr));
     }
-    SumScratch scratch = sum_scratch(stream, fused->grid);
-    std::vector<const void *> vals(static_cast<size_t>(nleaves) + 4);
-    std::vector<void *> args(static_cast<size_t>(nleaves) + 4);
-    for (int i = 0; i < nleaves; ++i) {
-      if (!leaves || !leaves[i]) fail(CS_INVALID_KERNEL_ARGS, "leaf " + std::to_string(i) + " is null");
-      vals[i] = leaves[i];
-    }
-    vals[nleaves] = scratch.partials;
-    vals[nleaves + 1] = scratch.ticket;
-    vals[nleaves + 2] = out;
-    vals[nleaves + 3] = out64;
-    for (size_t i = 0; i < vals.size(); ++i) args[i] = &vals[i];
-    CS_HIP(hipModuleLaunchKernel(fused->function, fused->grid, 1, 1, fused->block, 1, 1, 0, stream->stream,
-                                 args.data(), nullptr));
+    launch_finish_sum(stream, partials, fused->grid, out, out64);  // zero partials: an empty sum, 0
     CS_HIP(hipGetLastError());
   });
 }

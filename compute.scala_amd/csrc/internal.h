@@ -66,25 +66,40 @@ struct Slab {
   size_t in_use = 0;
 };
 
+// An event recorded on a stream (nullptr: the null stream) when a block was released.
+struct StreamEvent {
+  hipStream_t stream;
+  hipEvent_t event;
+};
+
 // A freed block waiting for the streams that may still read or write it.
 struct PendingFree {
   Block block;
-  std::vector<hipEvent_t> events;
+  std::vector<StreamEvent> events;
 };
 
 // Deterministic pooled arena over hipMalloc slabs. Allocation is first fit by
 // (slab creation order, offset); frees coalesce. A freed block becomes reusable once the
-// work queued on every stream of the device at release time has finished; before a new
-// slab is taken (or any reuse), pending frees are drained in release order, so an
-// identical sequence of calls yields identical addresses.
+// work queued on every stream of the device at release time has finished. Every allocation
+// first hands back all pending frees in release order, so an identical sequence of calls
+// yields identical addresses. How the allocation waits for them depends on the caller:
+//   * without a stream, the host waits for the release events (the block is free for any
+//     stream when the call returns);
+//   * for a stream (stream-ordered), nothing waits on the host: the stream is made to wait
+//     on the device for the release events of the other streams (its own work is ordered
+//     already), so the block may only be used on that stream, or after it.
 class Arena {
  public:
   explicit Arena(int ordinal) : ordinal_(ordinal) {}
   ~Arena();
   Block allocate(size_t bytes);
-  void release(Block b, std::vector<hipEvent_t> events);
+  Block allocate_on(size_t bytes, hipStream_t stream);
+  void release(Block b, std::vector<StreamEvent> events);
   // Waits for every pending free and hands its block back; the release events are destroyed.
-  void settle() { drain_pending(true); }
+  void settle() {
+    drain_pending(true);
+    reap_events(true);
+  }
   void trim();
   size_t reserved() const { return reserved_; }
   size_t in_use() const { return in_use_; }
@@ -92,11 +107,18 @@ class Arena {
 
  private:
   bool try_allocate(size_t bytes, Block *out);
+  Block take(size_t bytes);
   void drain_pending(bool wait);
+  // Hands back every pending free now; `stream` waits on the device for every release event
+  // of another stream that has not completed (kept in retired_ until it has).
+  void hand_back_ordered(hipStream_t stream);
+  // Destroys retired events that have completed (all of them, waiting, with `wait`).
+  void reap_events(bool wait);
   void give_back(const Block &b);
   int ordinal_;
   std::vector<std::unique_ptr<Slab>> slabs_;
   std::deque<PendingFree> pending_;
+  std::deque<StreamEvent> retired_;  // release events of handed-back blocks, until complete
   size_t reserved_ = 0;
   size_t in_use_ = 0;
 };
@@ -178,17 +200,12 @@ namespace csmi {
 // synchronizes the stream, so the first call of a given size must not be under capture.
 // The buffer it replaces stays allocated until the stream is destroyed.
 void *stream_scratch(cs_stream *stream, size_t bytes);
-// The arrival counter and room for n FP64 block partials of a ticketed sum (combine.h) in
-// the stream's scratch, and the launch that writes an empty sum (0) to *out and *out64
-// (each nullable).
-struct SumScratch {
-  unsigned *ticket;
-  double *partials;
-};
-SumScratch sum_scratch(cs_stream *stream, size_t n);
-void launch_empty_sum(cs_stream *stream, float *out, double *out64);
+// FP64 block partials of a sum in the stream's scratch (n of them), and the fixed-order FP64
+// combine that turns n partials into *out (FP32, nullable) and *out64 (nullable).
+double *sum_partials(cs_stream *stream, size_t n);
+void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, float *out, double *out64);
 // Events recorded now on every live stream of the device (for stream-ordered frees).
-std::vector<hipEvent_t> record_on_all_streams(cs_device *device);
+std::vector<StreamEvent> record_on_all_streams(cs_device *device);
 // Makes `device` current on the calling thread.
 void set_device(cs_device *device);
 // hiprtc build of `source` for the device's arch; throws Error with the log on failure.

@@ -7,10 +7,10 @@
 //   * per-thread sums are FP32 with a TwoSum compensation term, combined across the wave
 //     with xor shuffles and across the 4 waves through LDS, still as (sum, compensation)
 //     pairs, so the in-block result is accurate to about one FP32 rounding;
-//   * block results are combined in FP64 inside the same kernel: every block publishes an FP64
-//     partial with a write-through store and takes a ticket, and the block that arrives last
-//     adds all partials in a fixed order (combine.h), so results are deterministic run to
-//     run and there is no second launch;
+//   * block results are combined in FP64: every block writes an FP64 partial and a second,
+//     one-workgroup kernel adds the partials in a fixed order, so results are deterministic
+//     run to run. (A last-block ticket would need an agent-scope release fence in every
+//     block, which writes back the XCD's L2 each time; the extra launch is far cheaper.);
 //   * max uses fmaxf, which returns the non-NaN operand (OpenCLKernelBuilder.scala:279-285).
 // The reference's parallel program loses elements when the local size is not a power of
 // two (Tensors.scala:375-387); these kernels sum every element.
@@ -19,7 +19,6 @@
 #include <cmath>
 #include <cstdint>
 
-#include "combine.h"
 #include "internal.h"
 
 #pragma clang fp contract(off)
@@ -101,36 +100,65 @@ __device__ __forceinline__ void block_reduce(Acc &a, float &m) {
   }
 }
 
-// One workgroup of kFinish threads adds partials[0..n) in FP64 in the fixed order of
-// csmi_fixed_sum; used where no reduction kernel runs (an empty input: n = 0 writes 0).
+// One workgroup of kFinish threads adds partials[0..n) in FP64 in a fixed order: thread t
+// adds pairs 2t, 2t + 2 kFinish, ... (16-byte loads, eight in flight), then a fixed xor-shuffle
+// tree inside each wave and the 16 waves in order.
 constexpr int kFinish = 1024;
+typedef double d2 __attribute__((ext_vector_type(2)));
 __global__ __launch_bounds__(kFinish) void finish_sum_kernel(const double *__restrict__ partials, int64_t n,
                                                              float *out, double *out64) {
-  const double t = csmi_fixed_sum(partials, n, kFinish);
+  __shared__ double lw[kFinish / 64];
+  double acc = 0.0;
+  const int64_t pairs = n / 2;
+  const d2 *p2 = reinterpret_cast<const d2 *>(partials);  // partials are 16-byte aligned
+  int64_t i = threadIdx.x;
+  // eight 16-byte loads in flight per thread: one memory round trip for up to 16384 partials
+  for (; i + 7 * kFinish < pairs; i += 8 * kFinish) {
+    d2 v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = p2[i + k * kFinish];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc += v[k].x + v[k].y;
+  }
+  for (; i < pairs; i += kFinish) {
+    const d2 v = p2[i];
+    acc += v.x + v.y;
+  }
+  if (threadIdx.x == 0 && (n & 1)) acc += partials[n - 1];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+  if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = acc;
+  __syncthreads();
   if (threadIdx.x == 0) {
+    double t = lw[0];
+#pragma unroll
+    for (int w = 1; w < kFinish / 64; ++w) t += lw[w];
     if (out) *out = float(t);
     if (out64) *out64 = t;
   }
 }
 
-// The last block of a ticketed reduction: every partial of this launch is visible once the
-// acquire has run; the block adds them, writes the outputs and resets the counter.
-__device__ __forceinline__ void last_block_sum(const double *partials, int64_t n, unsigned *ticket, float *out,
-                                               double *out64) {
-  csmi_acquire_partials();
-  const double t = csmi_fixed_sum(partials, n, kBlock);
+__global__ __launch_bounds__(kFinish) void finish_max_kernel(const float *__restrict__ partials, int64_t n,
+                                                             float *out) {
+  __shared__ float lw[kFinish / 64];
+  float m = __builtin_nanf("");
+  for (int64_t i = threadIdx.x; i < n; i += kFinish) m = nanmax(m, partials[i]);
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) m = nanmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lw[threadIdx.x >> 6] = m;
+  __syncthreads();
   if (threadIdx.x == 0) {
-    if (out) *out = float(t);
-    if (out64) *out64 = t;
-    csmi_reset_ticket(ticket);
+    float t = lw[0];
+#pragma unroll
+    for (int w = 1; w < kFinish / 64; ++w) t = nanmax(t, lw[w]);
+    *out = t;
   }
 }
 
 // ----------------------------------------------------------------------------- full sum/max
 template <bool kMax>
 __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restrict__ in, int64_t n,
-                                                            double *partial64, float *partialmax,
-                                                            unsigned *ticket, float *out, double *out64) {
+                                                            double *partial64, float *partialmax) {
   Acc a;
   float m = __builtin_nanf("");
   const int64_t nvec = ((reinterpret_cast<uintptr_t>(in) & 15) == 0) ? n / 4 : 0;
@@ -164,21 +192,9 @@ __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restr
     if (kMax) m = nanmax(m, x); else a.add(x);
   }
   block_reduce(a, m);
-  __shared__ bool last;
-  if (threadIdx.x == 0)
-    last = kMax ? csmi_publish_f32(partialmax, blockIdx.x, m, ticket, gridDim.x)
-                : csmi_publish_f64(partial64, blockIdx.x, a.result64(), ticket, gridDim.x);
-  __syncthreads();
-  if (!last) return;
-  if (kMax) {
-    csmi_acquire_partials();
-    const float t = csmi_fixed_max(partialmax, gridDim.x, kBlock);
-    if (threadIdx.x == 0) {
-      *out = t;
-      csmi_reset_ticket(ticket);
-    }
-  } else {
-    last_block_sum(partial64, gridDim.x, ticket, out, out64);
+  if (threadIdx.x == 0) {
+    if (kMax) partialmax[blockIdx.x] = m;
+    else partial64[blockIdx.x] = a.result64();
   }
 }
 
@@ -188,8 +204,7 @@ __global__ __launch_bounds__(kBlock) void reduce_all_kernel(const float *__restr
 template <bool kVec>
 __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__restrict__ in, int64_t rows,
                                                              int64_t cols, float *row_sum, float *row_max,
-                                                             double *partial64, unsigned *ticket, float *total,
-                                                             double *total64) {
+                                                             double *partial64) {
   const int64_t row = blockIdx.x;
   const float *p = in + row * cols;
   Acc a;
@@ -226,15 +241,11 @@ __global__ __launch_bounds__(kBlock) void reduce_rows_kernel(const float *__rest
     }
   }
   block_reduce(a, m);
-  __shared__ bool last;
   if (threadIdx.x == 0) {
     if (row_sum) row_sum[row] = a.result();
     if (row_max) row_max[row] = m;
-    last = partial64 && csmi_publish_f64(partial64, static_cast<unsigned>(row), a.result64(), ticket,
-                                         static_cast<unsigned>(rows));
+    if (partial64) partial64[row] = a.result64();
   }
-  __syncthreads();
-  if (last) last_block_sum(partial64, rows, ticket, total, total64);
 }
 
 // ----------------------------------------------------------------------------- columns
@@ -324,10 +335,8 @@ __global__ __launch_bounds__(kBlock) void finish_cols_kernel(const double *__res
 
 }  // namespace
 
-// Per-stream scratch of a ticketed reduction: the arrival counter (zero between launches) in
-// the first 256 bytes, then the FP64 (or FP32 max) partials.
+// Per-stream scratch holding the FP64 (or FP32 max) partials.
 struct ReduceScratch {
-  unsigned *ticket;
   double *partial64;
   float *partialmax;
 };
@@ -339,20 +348,16 @@ using namespace csmi;
 static ReduceScratch scratch_for(cs_stream *stream, size_t partial_doubles) {
   char *base = static_cast<char *>(stream_scratch(stream, partial_doubles * 8 + 4096));
   ReduceScratch r;
-  r.ticket = reinterpret_cast<unsigned *>(base);
-  r.partial64 = reinterpret_cast<double *>(base + 256);
-  r.partialmax = reinterpret_cast<float *>(base + 256);
+  r.partial64 = reinterpret_cast<double *>(base);
+  r.partialmax = reinterpret_cast<float *>(base);
   return r;
 }
 
 namespace csmi {
-SumScratch sum_scratch(cs_stream *stream, size_t n) {
-  ReduceScratch r = scratch_for(stream, n);
-  return SumScratch{r.ticket, r.partial64};
-}
+double *sum_partials(cs_stream *stream, size_t n) { return scratch_for(stream, n).partial64; }
 
-void launch_empty_sum(cs_stream *stream, float *out, double *out64) {
-  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, nullptr, int64_t(0), out, out64);
+void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, float *out, double *out64) {
+  hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, partials, n, out, out64);
 }
 }  // namespace csmi
 
@@ -377,7 +382,9 @@ cs_status cs_reduce_sum(cs_stream *stream, const float *in, int64_t n, float *ou
     int grid = reduce_grid(stream, n);
     ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid));
     hipLaunchKernelGGL(reduce_all_kernel<false>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
-                       s.partial64, s.partialmax, s.ticket, out, out64);
+                       s.partial64, s.partialmax);
+    hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partial64,
+                       int64_t(grid), out, out64);
     CS_HIP(hipGetLastError());
   });
 }
@@ -392,7 +399,9 @@ cs_status cs_reduce_max(cs_stream *stream, const float *in, int64_t n, float *ou
     int grid = reduce_grid(stream, n);
     ReduceScratch s = scratch_for(stream, static_cast<size_t>(grid));
     hipLaunchKernelGGL(reduce_all_kernel<true>, dim3(grid), dim3(kBlock), 0, stream->stream, in, n,
-                       s.partial64, s.partialmax, s.ticket, out, nullptr);
+                       s.partial64, s.partialmax);
+    hipLaunchKernelGGL(finish_max_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partialmax,
+                       int64_t(grid), out);
     CS_HIP(hipGetLastError());
   });
 }
@@ -407,8 +416,10 @@ cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64
     set_device(stream->device);
     bool want_total = total || total64;
     if (rows == 0) {
-      // no rows: the total is an empty sum, 0
-      if (want_total) launch_empty_sum(stream, total, total64);
+      // no rows: the total is an empty sum, written by the combine over zero partials
+      if (want_total)
+        hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, nullptr, int64_t(0), total,
+                           total64);
       CS_HIP(hipGetLastError());
       return;
     }
@@ -416,7 +427,10 @@ cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64
     bool vec = cols % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
     auto k = vec ? reduce_rows_kernel<true> : reduce_rows_kernel<false>;
     hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(rows)), dim3(kBlock), 0, stream->stream, in, rows, cols,
-                       row_sum, row_max, want_total ? s.partial64 : nullptr, s.ticket, total, total64);
+                       row_sum, row_max, want_total ? s.partial64 : nullptr);
+    if (want_total)
+      hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partial64, rows, total,
+                         total64);
     CS_HIP(hipGetLastError());
   });
 }

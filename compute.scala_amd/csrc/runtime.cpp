@@ -95,27 +95,81 @@ void Arena::drain_pending(bool wait) {
   while (!pending_.empty()) {
     PendingFree &p = pending_.front();
     bool done = true;
-    for (hipEvent_t ev : p.events) {
+    for (const StreamEvent &se : p.events) {
       if (wait) {
-        CS_HIP(hipEventSynchronize(ev));
-      } else if (hipEventQuery(ev) != hipSuccess) {
+        CS_HIP(hipEventSynchronize(se.event));
+      } else if (hipEventQuery(se.event) != hipSuccess) {
         (void)hipGetLastError();
         done = false;
         break;
       }
     }
     if (!done) return;
-    for (hipEvent_t ev : p.events) (void)hipEventDestroy(ev);
+    for (const StreamEvent &se : p.events) (void)hipEventDestroy(se.event);
     give_back(p.block);
     pending_.pop_front();
   }
 }
 
+void Arena::hand_back_ordered(hipStream_t stream) {
+  // Every pending block goes back to the free lists now; its release events join the
+  // retired ones. A block handed back here may go to any later stream-ordered allocation, so
+  // the stream waits for every retired event that has not completed, not only for those of
+  // the blocks it takes (its own events need no wait: its work is ordered).
+  while (!pending_.empty()) {
+    PendingFree &p = pending_.front();
+    for (const StreamEvent &se : p.events) retired_.push_back(se);
+    give_back(p.block);
+    pending_.pop_front();
+  }
+  reap_events(false);
+  for (const StreamEvent &se : retired_) {
+    if (se.stream == stream) continue;
+    if (hipEventQuery(se.event) == hipSuccess) continue;
+    (void)hipGetLastError();
+    CS_HIP(hipStreamWaitEvent(stream, se.event, 0));
+  }
+}
+
+void Arena::reap_events(bool wait) {
+  while (!retired_.empty()) {
+    hipEvent_t ev = retired_.front().event;
+    if (wait) {
+      CS_HIP(hipEventSynchronize(ev));
+    } else if (hipEventQuery(ev) != hipSuccess) {
+      (void)hipGetLastError();
+      return;
+    }
+    (void)hipEventDestroy(ev);
+    retired_.pop_front();
+  }
+}
+
 Block Arena::allocate(size_t bytes) {
+  // Deterministic: every pending free is settled before the search, and the host waits for
+  // the release events of blocks handed back by stream-ordered allocations, so that every
+  // free block is free for any stream.
+  drain_pending(true);
+  reap_events(true);
+  return take(bytes);
+}
+
+Block Arena::allocate_on(size_t bytes, hipStream_t stream) {
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  CS_HIP(hipStreamIsCapturing(stream, &cap));
+  if (cap != hipStreamCaptureStatusNone) {
+    // a capture cannot wait on events recorded outside it: the host waits instead
+    drain_pending(true);
+    reap_events(true);
+  } else {
+    hand_back_ordered(stream);
+  }
+  return take(bytes);
+}
+
+Block Arena::take(size_t bytes) {
   if (bytes == 0) bytes = 1;
   bytes = round_up(bytes, kAlign);
-  // Deterministic: every pending free is settled before the search.
-  drain_pending(true);
   Block b;
   if (try_allocate(bytes, &b)) return b;
   size_t slab_bytes = std::max(kMinSlab, round_up(bytes, kSlabGranule));
@@ -143,13 +197,14 @@ Block Arena::allocate(size_t bytes) {
   return b;
 }
 
-void Arena::release(Block b, std::vector<hipEvent_t> events) {
+void Arena::release(Block b, std::vector<StreamEvent> events) {
   pending_.push_back(PendingFree{b, std::move(events)});
   drain_pending(false);
+  reap_events(false);
 }
 
 void Arena::trim() {
-  drain_pending(true);
+  settle();
   CS_HIP(hipSetDevice(ordinal_));
   for (auto it = slabs_.begin(); it != slabs_.end();) {
     if ((*it)->in_use == 0) {
@@ -162,13 +217,13 @@ void Arena::trim() {
   }
 }
 
-std::vector<hipEvent_t> record_on_all_streams(cs_device *device) {
-  std::vector<hipEvent_t> evs;
+std::vector<StreamEvent> record_on_all_streams(cs_device *device) {
+  std::vector<StreamEvent> evs;
   auto record = [&](hipStream_t s) {
     hipEvent_t ev;
     CS_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
     CS_HIP(hipEventRecord(ev, s));
-    evs.push_back(ev);
+    evs.push_back(StreamEvent{s, ev});
   };
   // A stream that is capturing a graph is skipped: an event recorded there would become a
   // node of the graph, and capture starts on an idle stream (cs_graph_begin_capture waits
@@ -678,6 +733,24 @@ cs_status cs_buffer_alloc(cs_device *device, size_t bytes, cs_buffer **buffer) {
     std::unique_ptr<cs_buffer> b(new cs_buffer());
     b->device = device;
     b->block = device->arena->allocate(bytes);
+    b->bytes = bytes;
+    *buffer = b.release();
+  });
+}
+
+cs_status cs_buffer_alloc_on(cs_stream *stream, size_t bytes, cs_buffer **buffer) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (!buffer) fail(CS_INVALID_VALUE, "buffer is null");
+    cs_device *device = stream->device;
+    if (bytes > device->props.totalGlobalMem)
+      fail(CS_INVALID_BUFFER_SIZE, "buffer of " + std::to_string(bytes) +
+                                       " bytes exceeds device memory");
+    set_device(device);
+    std::lock_guard<std::recursive_mutex> l(device->mu);
+    std::unique_ptr<cs_buffer> b(new cs_buffer());
+    b->device = device;
+    b->block = device->arena->allocate_on(bytes, stream->stream);
     b->bytes = bytes;
     *buffer = b.release();
   });

@@ -128,6 +128,12 @@ cs_status cs_event_on_complete(cs_event *event, void (*callback)(void *user), vo
  * carved from a pooled arena: the same sequence of alloc/release calls on one stream gives
  * the same device addresses. Blocks are 256-byte aligned. */
 cs_status cs_buffer_alloc(cs_device *device, size_t bytes, cs_buffer **buffer);   /* refcount 1 */
+/* Stream-ordered allocation (refcount 1): the same deterministic addresses as cs_buffer_alloc,
+ * but the host never waits for earlier releases; `stream` waits on the device for the work
+ * that other streams had queued when a reused block was released. The buffer is ready for
+ * work queued on `stream` (or ordered after it). Replaces the per-materialisation
+ * clCreateBuffer of allocateBuffer (OpenCL.scala:1397-1431) without its host round trip. */
+cs_status cs_buffer_alloc_on(cs_stream *stream, size_t bytes, cs_buffer **buffer);
 cs_status cs_buffer_retain(cs_buffer *buffer);
 cs_status cs_buffer_release(cs_buffer *buffer);
 cs_status cs_buffer_device_ptr(cs_buffer *buffer, void **device_ptr);
@@ -207,7 +213,9 @@ typedef struct cs_node {
 typedef enum cs_math_mode {
   CS_MATH_PRECISE = 0,  /* no FMA contraction, correctly rounded + - * / sqrt; exp, tanh and
                            log from the device math library (each within 2 ulp) */
-  CS_MATH_FAST = 1,     /* contraction and fast math, as -cl-unsafe-math-optimizations */
+  CS_MATH_FAST = 1,     /* as -cl-unsafe-math-optimizations (OpenCL.scala:1131-1135):
+                           contraction, reassociation, reciprocals, no signed zeros, and the
+                           hardware-rate exp and log; infinities and NaN keep their meaning */
   CS_MATH_ACCURATE = 2  /* as PRECISE, with exp and tanh within 1 ulp (compensated
                            polynomial evaluation; about twice the instructions) */
 } cs_math_mode;
@@ -400,6 +408,22 @@ cs_status cs_gemm(cs_stream *stream, int variant, int M, int N, int K,
 #define CS_CHECK_EXP_BOUNDED 114
 cs_status cs_math_check(cs_stream *stream, int op, int mode, uint32_t start, uint32_t count,
                         unsigned long long *stats_device);
+
+/* ---------------------------------------------------------------- HBM probe (measurement)
+ * A streaming pass with no arithmetic over n floats (n a multiple of 4, 16-byte aligned
+ * buffers): read src (kind READ), write dst (WRITE) or copy src to dst (COPY), with
+ * non-temporal loads and the given store form, items_per_thread (1, 2, 4 or 8) 16-byte
+ * vectors per thread in 256-thread workgroups, one vector of a wave contiguous per item (the
+ * generated kernels' layout). Its time is the ceiling a memory-bound kernel of the same size
+ * and shape can reach. No counterpart in the reference; the Tensor path never calls it. */
+#define CS_PROBE_READ 0
+#define CS_PROBE_WRITE 1
+#define CS_PROBE_COPY 2
+#define CS_PROBE_STORE_PLAIN 0
+#define CS_PROBE_STORE_NT 1
+#define CS_PROBE_STORE_NT_WRITE_THROUGH 2
+cs_status cs_hbm_probe(cs_stream *stream, int kind, int store, const float *src, float *dst, int64_t n,
+                       int items_per_thread);
 
 #ifdef __cplusplus
 }

@@ -150,25 +150,42 @@ def evaluate(closure, shape, leaves, mode="fp64"):
 
 
 def _ulp32(v):
-    """ulp of FP32 values (as FP64), ulp(0) = smallest subnormal."""
-    v = np.abs(np.asarray(v, dtype=np.float32))
-    nxt = np.nextafter(v, np.float32(np.inf))
-    return (nxt.astype(np.float64) - v.astype(np.float64))
+    """ulp of FP32 values (as FP64), ulp(0) = smallest subnormal; +inf for infinite or NaN
+    values (and magnitudes beyond FP32), whose neighbourhood has no finite width."""
+    v = np.abs(np.asarray(v, dtype=np.float64))
+    with np.errstate(all="ignore"):
+        f = v.astype(np.float32)
+        nxt = np.nextafter(f, np.float32(np.inf))
+        u = nxt.astype(np.float64) - f.astype(np.float64)
+    return np.where(np.isfinite(u), u, np.inf)
 
 
 # Error model of the fast math mode (the counterpart of -cl-unsafe-math-optimizations),
-# measured over every FP32 input by the device checker and stated here as bounds:
+# stated as bounds that the device measurements over every FP32 input sit inside:
 #   exp(x) = 2^(x * log2(e)) with the product rounded once: the rounding of the product
 #            (relative 2^-24, and log2(e) itself rounded) scales with |x|, so the result is
-#            within FAST_EXP_ULPS[0] + FAST_EXP_ULPS[1] * |x| ulp;
-#   log(x) = log2(x) * ln(2) from the hardware base-2 logarithm: within FAST_LOG_ABS
-#            absolute plus FAST_LOG_ULPS ulp (near x = 1, where log is small, the absolute
-#            term dominates);
-#   tanh keeps the precise mode's function (2 ulp), sqrt may use the hardware square root.
-FAST_EXP_ULPS = (2.0, 2.0)
-FAST_LOG_ABS = 2.0 ** -21
+#            within FAST_EXP_ULPS[0] + FAST_EXP_ULPS[1] * |x| ulp (the rounded product alone can
+#            cost 1.25 |x| ulp; measured on the device: at most 1 ulp
+#            for |x| < 1, 59 for |x| < 64, 64 up to the overflow at 88.7); results below
+#            2^-126 may be flushed to zero (FAST_FLUSH absolute);
+#   log(x) from the hardware base-2 logarithm: within FAST_LOG_ULPS ulp everywhere
+#            (measured 2, subnormal arguments included);
+#   sqrt from the hardware square root: FAST_SQRT_ULPS ulp, and a subnormal argument may
+#            be flushed to a zero of its sign first (an error below 2^-63, and -0 rather
+#            than NaN for a negative one);
+#   tanh keeps the precise mode's function (2 ulp).
+FAST_EXP_ULPS = (2.0, 1.5)
+FAST_FLUSH = 2.0 ** -126
+FAST_LOG_ABS = 0.0
 FAST_LOG_ULPS = 2.0
 FAST_SQRT_ULPS = 2.0
+
+
+def _times(x, y):
+    """x * y for propagated bounds, where a zero factor gives 0 even against an infinite one
+    (an exact input of an operation whose derivative overflows adds no error)."""
+    with np.errstate(all="ignore"):
+        return np.where((np.asarray(x) == 0) | (np.asarray(y) == 0), 0.0, np.asarray(x, np.float64) * y)
 
 
 def evaluate_with_bound(closure, shape, leaves, ulps=2.0, mode="precise"):
@@ -233,11 +250,13 @@ def evaluate_with_bound(closure, shape, leaves, ulps=2.0, mode="precise"):
                     own = (ulps + 0.5) * _ulp32(v)
                     if fast and node.op == "exp":
                         own = (FAST_EXP_ULPS[0] + FAST_EXP_ULPS[1] * (np.abs(a64) + ba) + 0.5) * _ulp32(v)
+                        own = own + FAST_FLUSH
                     elif fast and node.op == "log":
                         own = FAST_LOG_ABS + (FAST_LOG_ULPS + 0.5) * _ulp32(v)
                     elif fast and node.op == "sqrt":
                         own = (FAST_SQRT_ULPS + 0.5) * _ulp32(v)
-                    out = (v, deriv * ba + own)
+                        own = own + np.where(np.abs(a64) < FAST_FLUSH, 2.0 ** -63, 0.0)
+                    out = (v, _times(deriv, ba) + own)
         elif kind == "Binary":
             a, ba = ev(node.lhs)
             b, bb = ev(node.rhs)
@@ -248,9 +267,9 @@ def evaluate_with_bound(closure, shape, leaves, ulps=2.0, mode="precise"):
                 if node.op in ("+", "-"):
                     prop = ba + bb
                 elif node.op == "*":
-                    prop = np.abs(a64) * bb + np.abs(b64) * ba + ba * bb
+                    prop = _times(np.abs(a64), bb) + _times(np.abs(b64), ba) + _times(ba, bb)
                 elif node.op == "/":
-                    prop = (ba + np.abs(a64 / b64) * bb) / np.maximum(np.abs(b64) - bb, 1e-300)
+                    prop = (ba + _times(np.abs(a64 / b64), bb)) / np.maximum(np.abs(b64) - bb, 1e-300)
                 elif node.op in ("min", "max"):
                     prop = np.maximum(ba, bb)
                 else:

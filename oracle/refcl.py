@@ -371,7 +371,7 @@ class Kernel(object):
         self.source, self.leaves, params, out_type, self.width = generate(closure, self.shape)
         self.total = len(self.shape)
         code = _PRELUDE + self.source + _driver(params, out_type, self.total)
-        digest = hashlib.sha1((code + mode).encode()).hexdigest()[:20]
+        digest = hashlib.sha1((code + mode + "|compile, then link without the mode flags").encode()).hexdigest()[:20]
         os.makedirs(BUILD, exist_ok=True)
         so = os.path.join(BUILD, digest + ".so")
         with _lock:
@@ -379,9 +379,17 @@ class Kernel(object):
                 c = os.path.join(BUILD, digest + ".c")
                 with open(c, "w") as f:
                     f.write(code)
-                cmd = ["gcc", "-O2", "-std=gnu11", "-fopenmp", "-fPIC", "-shared", "-w"] + MODES[mode] + \
-                    ["-o", so + ".tmp", c, "-lm"]
+                # compile with the mode's flags, link without them: gcc links crtfastmath.o into
+                # a shared object built with -funsafe-math-optimizations, whose constructor sets
+                # flush-to-zero and denormals-are-zero for the whole process on load (and the
+                # FP64 oracle in the same process would then flush subnormals)
+                obj = os.path.join(BUILD, digest + ".o")
+                cmd = ["gcc", "-O2", "-std=gnu11", "-fopenmp", "-fPIC", "-c", "-w"] + MODES[mode] + ["-o", obj, c]
                 proc = subprocess.run(cmd, capture_output=True, text=True)
+                if proc.returncode != 0:
+                    raise BuildProgramFailure(proc.stderr[-4000:])
+                link = ["gcc", "-shared", "-fopenmp", "-o", so + ".tmp", obj, "-lm"]
+                proc = subprocess.run(link, capture_output=True, text=True)
                 if proc.returncode != 0:
                     raise BuildProgramFailure(proc.stderr[-4000:])
                 os.replace(so + ".tmp", so)

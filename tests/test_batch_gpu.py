@@ -72,3 +72,41 @@ def test_batch_of_one_and_tiny(gpu_tensors):
             capi.lib.cs_fused_release(f)
     finally:
         eset.release()
+
+
+def test_c5_full_size(gpu_tensors):
+    """The batched workload at its stated size: 512 expressions of seed 5001 (3-12 ops, 1e3 to
+    4e6 elements, permute and broadcast views), bodies V=2, U=2 in 512-thread workgroups. The
+    graph-captured persistent batch gives the same bits as per-expression launches of the
+    same kernels for all 512 outputs, and every 16th output is within the propagated bound of
+    the FP32 program with correctly rounded operations."""
+    from compute import _capi as capi
+    T = gpu_tensors
+    exprs = workloads.uneven_dags(T, 512, 5001)
+    print("c5: %d expressions, %.2f GB" % (len(exprs), sum(e.bytes for e in exprs) / 1e9), flush=True)
+    eset = bm.ExpressionSet(T, exprs)
+    try:
+        cfg = {"vector_width": 2, "items_per_thread": 2, "block_size": 512, "nontemporal_store": 1}
+        fused = bm.compile_many(T, exprs, [cfg] * len(exprs))
+        for i, f in enumerate(fused):
+            eset.launch(i, f, 512)
+        each = eset.outputs()
+        eset.clear_outputs()
+        b = bm.Batch(eset, fused, block=512)
+        g = bm.Graph(T, b.launch)
+        eset.clear_outputs()
+        g.launch()
+        T.synchronize()
+        got = eset.outputs()
+        same = [np.array_equal(got[i].view(np.uint32), each[i].view(np.uint32)) for i in range(len(exprs))]
+        assert all(same), [i for i, s in enumerate(same) if not s][:10]
+        g.release()
+        b.release()
+        for f in fused:
+            capi.lib.cs_fused_release(f)
+        for i in range(0, len(exprs), 16):
+            e = exprs[i]
+            value, bound = fp64.evaluate_with_bound(e.tensor.closure, e.shape, e.leaf_data, ulps=2.0)
+            assert fp64.within_bound(each[i], value, bound).all(), i
+    finally:
+        eset.release()

@@ -75,3 +75,19 @@ def test_fast_exp_and_division_are_inside_the_fast_bound():
     by_recip = (p * recip).astype(np.float32)
     value, bound = fp64.evaluate_with_bound(d.closure, d.shape, [p, q], mode="fast")
     assert fp64.within_bound(by_recip, value, bound).all()
+
+
+def test_bound_through_an_overflowing_exp():
+    """exp(100) overflows to +inf in FP32 and tanh(inf) is exactly 1: the propagated bound is
+    a number (infinite where the derivative overflows), never NaN from 0 * inf, in both
+    modes; an exact input adds no error to an op whose derivative overflows."""
+    T = _tensors()
+    x = np.array([100.0, 1.0, -100.0], np.float32)
+    e = T.Tensor.tanh(T.Tensor.exp(T.Tensor(x)))
+    for mode in ("precise", "fast"):
+        value, bound = fp64.evaluate_with_bound(e.closure, e.shape, [x], mode=mode)
+        assert not np.isnan(bound).any(), mode
+        assert value[0] == 1.0 and bound[1] < 1e-5
+    z = T.Tensor.exp(T.Tensor(x)) * T.Tensor(np.zeros(3, np.float32))
+    value, bound = fp64.evaluate_with_bound(z.closure, z.shape, [x, np.zeros(3, np.float32)])
+    assert np.isnan(value[0]) and not np.isnan(bound[1:]).any()

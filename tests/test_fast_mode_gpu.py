@@ -34,20 +34,26 @@ def _binade_allowance_exp(e):
 def test_fast_functions_over_every_float(fast, op):
     """__expf and __logf, the functions the fast mode emits, over all 2^32 inputs, one binade
     (2^23 inputs of one sign and exponent) per check, against FP64 rounded to FP32: exp within
-    the model's allowance at the binade's largest |x|; log within FAST_LOG_ULPS + 0.5 ulp
-    outside [0.5, 2) and within FAST_LOG_ABS absolute plus that inside (checked separately,
-    where log is small)."""
+    the model's allowance at the binade's largest |x| wherever its result is a normal number
+    (below 2^-126 it may flush to zero, which the sampled test checks against FAST_FLUSH);
+    log within FAST_LOG_ULPS ulp everywhere, NaN for negative arguments."""
     T = fast
     stats = alloc(T, 3, np.uint64)
     zero = np.zeros(3, np.uint64)
     code = {"exp": capi.OP_EXP, "log": capi.OP_LOG}[op]
+    flush_at = int(np.array([np.log(2.0 ** -126)], np.float32).view(np.uint32)[0])  # exp's result there is 2^-126
     worst = {}
     for sign in (0, 1):
         for e in range(256):
+            start = (sign << 31) | (e << 23)
+            count = 1 << 23
+            if op == "exp" and sign == 1 and start < flush_at < start + count:
+                count = flush_at - start  # arguments whose exp is a normal number
+            elif op == "exp" and sign == 1 and start >= flush_at and e < 255:
+                continue  # every result below 2^-126
             capi.check(capi.lib.cs_memcpy_htod(T.stream, ctypes.c_void_p(stats.ptr),
                                                zero.ctypes.data_as(ctypes.c_void_p), 24))
-            capi.check(capi.lib.cs_math_check(T.stream, code, capi.MATH_FAST, (sign << 31) | (e << 23), 1 << 23,
-                                              stats.ptr))
+            capi.check(capi.lib.cs_math_check(T.stream, code, capi.MATH_FAST, start, count, stats.ptr))
             s = download(T, stats, 3, np.uint64)
             worst[(sign, e - 127)] = int(s[0])
             x = np.array([s[2]], np.uint32).view(np.float32)[0]
@@ -59,8 +65,8 @@ def test_fast_functions_over_every_float(fast, op):
             else:
                 if sign == 1 and e > 0:
                     assert s[0] == 0, x  # negative arguments: NaN (the zero binade holds -0)
-                elif e - 127 not in (-1, 0):
-                    assert s[0] <= fp64.FAST_LOG_ULPS + 0.5, (sign, e - 127, int(s[0]), x)
+                else:
+                    assert s[0] <= fp64.FAST_LOG_ULPS, (sign, e - 127, int(s[0]), x)
     print("fast %s: worst ulp per binade (sign, exponent): %s" % (op, sorted(worst.items(), key=lambda kv: -kv[1])[:8]))
     stats.release()
 
@@ -86,6 +92,12 @@ def test_fast_generated_kernels_sampled(fast, op):
     e = fn(T.Tensor(x))
     got = e.flatArray()
     value, bound = fp64.evaluate_with_bound(e.closure, e.shape, [x], mode="fast")
+    if op == "sqrt":
+        # a subnormal argument may be flushed to a zero of its sign first: sqrt(-tiny) is then
+        # -0 instead of NaN
+        flushed = (x < 0) & (x > -fp64.FAST_FLUSH)
+        assert np.all(np.isnan(got[flushed]) | (got[flushed] == 0))
+        got, value, bound, x = got[~flushed], value[~flushed], bound[~flushed], x[~flushed]
     ok = fp64.within_bound(got, value, bound)
     assert ok.all(), (op, x[~ok][:5], got[~ok][:5], value[~ok][:5])
     assert np.array_equal(np.isnan(got), np.isnan(value)) and np.array_equal(np.isinf(got), np.isinf(value))

@@ -86,3 +86,58 @@ def test_empty_inner_dimension(gpu_tensors, variant):
     c = download(T, bc, M * N)
     assert np.array_equal(c, np.zeros(M * N, np.float32))
     bc.release()
+
+
+def test_idioms_known_answer_both_routes():
+    """TensorsSpec's two matrix-multiply idioms print [[74.0,...],[173.0,...]] through the
+    reference's own kernels (small: below the thresholds) and through the GEMM route."""
+    from compute import Tensors
+    from test_gemm_idioms import mm1, mm2
+    for route in ("large", "always"):
+        with Tensors(0, gemm_idioms=route) as T:
+            m1, m2 = T.Tensor(spec_cases.M1), T.Tensor(spec_cases.M2)
+            before = T.idiom_gemms
+            for f in (mm1, mm2):
+                assert f(T, m1, m2).toString() == spec_cases.GOLDEN["matrix_multiplication"]["expect"], (route, f)
+            assert T.idiom_gemms - before == (2 if route == "always" else 0)
+
+
+def test_mm2_4096_is_one_gemm():
+    """matrixMultiply2 written as the reference writes it, at 4096^3 (its [i,j,k] product
+    would be 256 GiB): one cs_gemm call, 512 rows of C against FP64 within the FP32 bound."""
+    from compute import Tensors
+    from test_gemm_idioms import mm2
+    n = 4096
+    a, b = inputs.c4_inputs(n)
+    with Tensors(0) as T:
+        ta, tb = T.Tensor(a), T.Tensor(b)
+        before = (T.launches, T.idiom_gemms)
+        c = mm2(T, ta, tb).flatArray().reshape(n, n)
+        assert (T.launches - before[0], T.idiom_gemms - before[1]) == (1, 1)
+    rows = np.arange(5, n, 8)
+    err = fp64.gemm_normalized_error(c[rows], a[rows], b)
+    print("matrixMultiply2 at 4096^3 through the GEMM: normalized error %.3g" % err)
+    assert err <= BOUND["fp32"]
+
+
+def test_mm1_is_one_gemm():
+    """matrixMultiply1 with 64 x 128 = 8192 unrolled products per output row: one cs_gemm
+    call, within the FP32 bound of FP64; below the threshold (8 x 8) the reference's fused
+    kernel, bit-exact with its FP32 left fold."""
+    from compute import Tensors
+    from test_gemm_idioms import mm1
+    a = inputs.uniform(1024 * 64, 4011).reshape(1024, 64)
+    b = inputs.uniform(64 * 128, 4012).reshape(64, 128)
+    with Tensors(0) as T:
+        before = T.idiom_gemms
+        c = mm1(T, T.Tensor(a), T.Tensor(b)).flatArray().reshape(1024, 128)
+        assert T.idiom_gemms - before == 1
+        err = fp64.gemm_normalized_error(c, a, b)
+        assert err <= BOUND["fp32"], err
+        a8, b8 = a[:, :8].copy(), b[:8, :8].copy()
+        c8 = mm1(T, T.Tensor(a8), T.Tensor(b8)).flatArray().reshape(1024, 8)
+        assert T.idiom_gemms - before == 1
+    want = np.zeros((1024, 8), np.float32)
+    for j in range(8):
+        want = (want + a8[:, j:j + 1] * b8[j:j + 1, :]) if j else a8[:, :1] * b8[:1, :]
+    assert np.array_equal(c8.view(np.uint32), want.view(np.uint32))

@@ -177,3 +177,18 @@ def test_error_bound_holds_for_host_libm_and_catches_planted_errors(seed, offlin
     planted = (th + bad_ex) - lg * sq
     ok = fp64.within_bound(planted, value, bound)
     assert not ok.all()
+
+
+def test_fast_kernels_leave_the_fp_environment_alone():
+    """Loading a refcl-host kernel built in the fast mode must not switch the process to
+    flush-to-zero / denormals-are-zero (gcc links crtfastmath.o into shared objects built with
+    -funsafe-math-optimizations): the FP64 oracle in the same process needs subnormals."""
+    import numpy as np
+    from compute import Tensors
+    from oracle import refcl
+    T = Tensors(offline=True)
+    x = np.array([1e-41, 2.0], np.float32)
+    e = T.Tensor(x) * T.Tensor(x)
+    refcl.Kernel(e.closure, e.shape, "fast")([x, x])
+    tiny = np.array([1e-41], np.float32)
+    assert np.sqrt(tiny)[0] > 0 and (tiny * np.float32(2.0))[0] > 0
