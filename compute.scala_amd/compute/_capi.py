@@ -226,6 +226,7 @@ _PROTOS = {
     "cs_reduce_sum": (I, [P, P, I64, P, P]),
     "cs_reduce_max": (I, [P, P, I64, P]),
     "cs_reduce_rows": (I, [P, P, I64, I64, P, P, P, P]),
+    "cs_reduce_rows_noalloc": (I, [P, P, I64, I64, P, P, P, P]),
     "cs_reduce_cols": (I, [P, P, I64, I64, P]),
     "cs_random": (I, [P, P, I64, U32]),
     "cs_random_normal": (I, [P, P, I64, U32]),

@@ -361,6 +361,32 @@ void launch_finish_sum(cs_stream *stream, const double *partials, int64_t n, flo
 }
 }  // namespace csmi
 
+// ----------------------------------------------------------------------------- no allocation
+// cs_reduce_rows_noalloc keeps its row partials in this file-scope device array (one copy per
+// GPU, held until the process ends). Host code reaches it only through the address
+// hipGetSymbolAddress returns on each device.
+constexpr int64_t kNoallocMaxRows = int64_t(1) << 20;
+static __device__ __align__(256) double scratch[kNoallocMaxRows];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::map<int, double *> g_noalloc_scratch;  // device ordinal -> address of `scratch` there
+
+// The device address of `scratch` on `device`, looked up once per device with that device
+// current.
+double *noalloc_scratch(cs_device *device) {
+  std::lock_guard<std::mutex> l(g_noalloc_mu);
+  auto it = g_noalloc_scratch.find(device->ordinal);
+  if (it != g_noalloc_scratch.end()) return it->second;
+  set_device(device);
+  void *p = nullptr;
+  CS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) fail(CS_OUT_OF_RESOURCES, "no device address for the reduction scratch");
+  g_noalloc_scratch[device->ordinal] = static_cast<double *>(p);
+  return static_cast<double *>(p);
+}
+}  // namespace
+
 static int reduce_grid(cs_stream *stream, int64_t n) {
   int cus = stream->device->props.multiProcessorCount;
   int64_t want = (n / 4 + kBlock * 4 - 1) / (kBlock * 4);  // >= 4 vectors per thread
@@ -430,6 +456,39 @@ cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64
                        row_sum, row_max, want_total ? s.partial64 : nullptr);
     if (want_total)
       hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, s.partial64, rows, total,
+                         total64);
+    CS_HIP(hipGetLastError());
+  });
+}
+
+cs_status cs_reduce_rows_noalloc(cs_stream *stream, const float *in, int64_t rows, int64_t cols, float *row_sum,
+                                 float *row_max, float *total, double *total64) {
+  return guard([&] {
+    if (!stream) fail(CS_INVALID_VALUE, "stream is null");
+    if (rows < 0 || cols < 0) fail(CS_INVALID_VALUE, "negative shape");
+    if (rows > 0x7fffffff) fail(CS_INVALID_VALUE, "more than 2^31-1 rows");
+    if (rows > 0 && cols > 0 && !in) fail(CS_INVALID_VALUE, "input is null");
+    const bool want_total = total || total64;
+    if (want_total && rows > kNoallocMaxRows)
+      fail(CS_INVALID_BUFFER_SIZE, "cs_reduce_rows_noalloc takes at most " + std::to_string(kNoallocMaxRows) +
+                                       " rows when a total is asked for, got " + std::to_string(rows));
+    set_device(stream->device);
+    if (rows == 0) {
+      // no rows: the total is an empty sum, written by the combine over zero partials
+      if (want_total)
+        hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, nullptr, int64_t(0), total,
+                           total64);
+      CS_HIP(hipGetLastError());
+      return;
+    }
+    // every partial the combine reads is written by this call's row kernel first
+    double *partial64 = want_total ? noalloc_scratch(stream->device) : nullptr;
+    const bool vec = cols % 4 == 0 && (reinterpret_cast<uintptr_t>(in) & 15) == 0;
+    auto k = vec ? reduce_rows_kernel<true> : reduce_rows_kernel<false>;
+    hipLaunchKernelGGL(k, dim3(static_cast<unsigned>(rows)), dim3(kBlock), 0, stream->stream, in, rows, cols,
+                       row_sum, row_max, partial64);
+    if (want_total)
+      hipLaunchKernelGGL(finish_sum_kernel, dim3(1), dim3(kFinish), 0, stream->stream, partial64, rows, total,
                          total64);
     CS_HIP(hipGetLastError());
   });

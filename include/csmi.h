@@ -365,6 +365,15 @@ cs_status cs_reduce_max(cs_stream *stream, const float *in, int64_t n, float *ou
  * (README.md:342, TensorsSpec.scala:477). */
 cs_status cs_reduce_rows(cs_stream *stream, const float *in, int64_t rows, int64_t cols,
                          float *row_sum, float *row_max, float *total, double *total64);
+/* cs_reduce_rows without allocating: no counterpart in the reference. For callers that
+ * cannot allocate device memory (code under graph capture, serving loops, hosts that forbid
+ * hipMalloc). The same kernels and results as cs_reduce_rows; the FP64 row partials of the
+ * total live in a fixed device array of the library (8 MiB per GPU, held until the process
+ * ends), so a total (total or total64 non-null) is limited to 1048576 rows
+ * (CS_INVALID_BUFFER_SIZE beyond). Calls on one GPU share that array: calls that overlap in
+ * time on that GPU (on different streams) race and must not be made. */
+cs_status cs_reduce_rows_noalloc(cs_stream *stream, const float *in, int64_t rows, int64_t cols,
+                                 float *row_sum, float *row_max, float *total, double *total64);
 /* Column sums of a row-major [rows, cols] matrix (the `split(0).reduce(_ + _)` idiom). */
 cs_status cs_reduce_cols(cs_stream *stream, const float *in, int64_t rows, int64_t cols,
                          float *col_sum);
