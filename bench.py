@@ -580,12 +580,13 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     each_out = eset.outputs()
     eset.clear_outputs()
     # The persistent batch runs in 512-thread workgroups; its bodies' V and U per expression
-    # come from the cost model (block size fixed at the batch's 512), among the bodies whose
-    # V*U <= 4: the batch's occupancy is set by its largest body, and these stay within 64
-    # VGPRs. Beside it: the fixed V = 2, U = 2 bodies, and the per-expression best of the
-    # exhaustive timing database at 512 threads (the autotuner's choice, the judge of the
-    # model's), each as its own batch.
-    light = [c for c in cfgs if c["block_size"] == 512 and c["vector_width"] * c["items_per_thread"] <= 4]
+    # come from the cost model (block size fixed at the batch's 512), among the bodies with
+    # V*U = 4: every work item then covers 2048 elements (the queue stays balanced) and the
+    # batch's occupancy, set by its largest body, stays at that of 4 elements per thread.
+    # Beside it: the fixed V = 2, U = 2 bodies, and the per-expression best of the exhaustive
+    # timing database at 512 threads among the same bodies (the autotuner's choice, the judge
+    # of the model's), each as its own batch.
+    light = [c for c in cfgs if c["block_size"] == 512 and c["vector_width"] * c["items_per_thread"] == 4]
     choices = {"fixed": [(batch_vu[0], batch_vu[1])] * len(exprs)}
     model = cm.load(T, model_path)
     if model is not None:
@@ -628,7 +629,7 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     if "exhaustive" in batches and "model" in batches:
         res["batch_model_regret_vs_exhaustive"] = batches["model"]["us"] / batches["exhaustive"]["us"] - 1.0
     res["batch_config"] = {"block_size": 512, "nontemporal_store": nontemporal,
-                           "bodies": "V, U per expression from the cost model (V*U <= 4)" if use == "model"
+                           "bodies": "V, U per expression from the cost model (V*U = 4)" if use == "model"
                            else "V=%d, U=%d" % tuple(batch_vu)}
     res["per_expression_default_us"] = us_def
     res["per_expression_us"] = us_each
@@ -733,16 +734,22 @@ def bench_split(T, capi, n=64, seed=6001, out_dir=None):
             grp.append(k)
     model = cm.SplitModel(T, len(X[0]), hidden=32, layers=2, seed=5)
     hist = model.fit(np.stack(X), np.array(y), epochs=400, batch=32, lr=2e-3, groups=np.array(grp))
+    # the margin a cut must be predicted to win by, chosen on the training joins
+    preds = {k: model.predicted(joins[k].tensor) for k in range(n)}
+    margins = (0.0, 0.02, 0.05, 0.1, 0.2, 0.5)
+    train_t = [sum(times[(k, cm.pick_groups(*preds[k], m))] for k in range(ntrain)) for m in margins]
+    model.margin = margins[int(np.argmin(train_t))]
     best = chosen = never = 0.0
     picks = {}
     for k in range(ntrain, n):
         j = joins[k]
         cands = autotune.split_candidates(j.parts)
         best += min(times[(k, g)] for g in cands)
-        picks[k] = model.choose_groups(j.tensor)
+        picks[k] = cm.pick_groups(*preds[k], model.margin)
         chosen += times[(k, picks[k])]
         never += times[(k, 1)]
-    res = {"joins": n, "seed": seed, "timings": len(times), "held_out": n - ntrain,
+    res = {"joins": n, "seed": seed, "timings": len(times), "held_out": n - ntrain, "margin": model.margin,
+           "held_out_picks_split": sum(1 for k in picks if picks[k] > 1),
            "train_loss_first_last": [hist[0], hist[-1]],
            "held_out_us": {"exhaustive": best, "model": chosen, "one_kernel": never},
            "regret_vs_exhaustive": chosen / best - 1.0, "gain_over_one_kernel": never / chosen,

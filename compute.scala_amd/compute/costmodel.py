@@ -236,15 +236,30 @@ def split_features(join, groups):
 
 class SplitModel(CostModel):
     """The cost model of fusion splits: predicts the log-time of a join cut into g kernels
-    (relative to the join's mean, as CostModel with groups) and picks g."""
+    (relative to the join's mean, as CostModel with groups) and picks g. A cut is taken only
+    when it is predicted faster than one kernel by more than `margin` (a fraction, calibrated
+    on the training joins): one kernel is the reference's method and usually the fastest."""
 
-    def choose_groups(self, join, largest=8):
+    margin = 0.0
+
+    def predicted(self, join, largest=8):
         from . import autotune
         cands = autotune.split_candidates(len(join._parts), largest)
-        if len(cands) == 1:
-            return cands[0]
         X = np.stack([split_features(join, g) for g in cands])
-        return cands[int(np.argmin(self.predict_log_us(X)))]
+        return cands, self.predict_log_us(X)
+
+    def choose_groups(self, join, largest=8):
+        cands, pred = self.predicted(join, largest)
+        return pick_groups(cands, pred, self.margin)
+
+
+def pick_groups(cands, pred, margin):
+    """The cut with the lowest predicted log-time, if it beats one kernel (cands[0] == 1) by
+    more than `margin`; otherwise one kernel."""
+    best = int(np.argmin(pred))
+    if cands[best] != 1 and pred[best] < pred[0] - math.log1p(margin):
+        return cands[best]
+    return 1
 
 
 SPLIT_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "splitmodel.npz")
@@ -253,7 +268,8 @@ SPLIT_MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dat
 def save(model, path):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     np.savez(path, params=model.net.get_params(), mean=model.mean, std=model.std,
-             ymean=np.array([model.ymean]), widths=np.array(model.net.widths))
+             ymean=np.array([model.ymean]), widths=np.array(model.net.widths),
+             margin=np.array([getattr(model, "margin", 0.0)]))
 
 
 def load_split_model(tensors, path=SPLIT_MODEL_PATH):
@@ -264,4 +280,6 @@ def load_split_model(tensors, path=SPLIT_MODEL_PATH):
     model = SplitModel(tensors, int(d["widths"][0]), hidden=int(d["widths"][1]), layers=len(d["widths"]) - 2)
     model.net.set_params(d["params"])
     model.mean, model.std, model.ymean = d["mean"], d["std"], float(d["ymean"][0])
+    if "margin" in d:
+        model.margin = float(d["margin"][0])
     return model
