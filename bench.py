@@ -7,8 +7,11 @@ Headline (the `value` of the JSON line): the view-heavy fused chain
 as one generated kernel, in GB/s of algorithmic traffic (X read once, bias, out written:
 33,555,456 bytes per launch). One step = one launch over one input set; 16 input/output
 sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch streams
-from HBM. The K timed launches are replayed from a captured hipGraph and bracketed by HIP
-events on the launch stream.
+from HBM. The K timed launches are spread over a pool of 4 streams (set i on stream i mod 4,
+as the reference's CommandQueuePool leases a queue per command), each replaying captured
+hipGraphs of its sets, and bracketed by HIP events on the launch stream that every stream
+waits for and waits on; the same launches one after another on one stream are reported
+beside it (config.one_stream).
 
 The same line carries the other workloads under "also": a*b+c on [1024,1024] and its sum,
 the one-pass row sum/row max/total over [16384,4096], and C = A.B at 4096^3 per GEMM
@@ -98,7 +101,75 @@ def capture(capi, stream, fns):
     return g
 
 
-def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
+class QueuePool(object):
+    """Launches of independent input sets spread over `n` streams, set i on stream i mod n,
+    as the reference's CommandQueuePool leases one of its queues to each command
+    (OpenCL.scala:423-448, 1313-1322; five per device in gpu.scala:26). Each stream replays
+    graphs of its own sets, so one launch's ramp-down overlaps the next one's start. A run is
+    bracketed by an event on the launch stream that every stream waits for and by events of
+    every stream that the launch stream waits for, so HIP events on the launch stream time it."""
+
+    def __init__(self, T, capi, launch_on, sets, n=4):
+        self.T, self.capi, self.lib = T, capi, capi.lib
+        self.launch_on, self.sets, self.n = launch_on, sets, n
+        self.streams = []
+        for _ in range(n):
+            st = ctypes.c_void_p()
+            capi.check(self.lib.cs_stream_create(T.device, ctypes.byref(st)))
+            self.streams.append(st)
+        self.full = [self._graph(k, range(sets)) for k in range(n)]
+        self.tails = {}
+
+    def _graph(self, k, indices):
+        mine = [i for i in indices if i % self.n == k]
+        if not mine:
+            return None
+        st = self.streams[k]
+        for i in mine:
+            self.launch_on(st, i)
+        self.capi.check(self.lib.cs_stream_synchronize(st))
+        self.capi.check(self.lib.cs_graph_begin_capture(st))
+        for i in mine:
+            self.launch_on(st, i)
+        g = ctypes.c_void_p()
+        self.capi.check(self.lib.cs_graph_end_capture(st, ctypes.byref(g)))
+        return g
+
+    def run(self, count):
+        """Exactly `count` launches: whole rounds of the sets, then sets 0 .. count mod sets - 1."""
+        lib, check = self.lib, self.capi.check
+        rem = count % self.sets
+        if rem and rem not in self.tails:
+            self.tails[rem] = [self._graph(k, range(rem)) for k in range(self.n)]
+        e0 = ctypes.c_void_p()
+        check(lib.cs_event_record(self.T.stream, ctypes.byref(e0)))
+        for st in self.streams:
+            check(lib.cs_stream_wait_event(st, e0))
+        for _ in range(count // self.sets):
+            for st, g in zip(self.streams, self.full):
+                check(lib.cs_graph_launch(st, g))
+        if rem:
+            for st, g in zip(self.streams, self.tails[rem]):
+                if g is not None:
+                    check(lib.cs_graph_launch(st, g))
+        for st in self.streams:
+            e = ctypes.c_void_p()
+            check(lib.cs_event_record(st, ctypes.byref(e)))
+            check(lib.cs_stream_wait_event(self.T.stream, e))
+            lib.cs_event_release(e)
+        lib.cs_event_release(e0)
+
+    def release(self):
+        for gs in [self.full] + list(self.tails.values()):
+            for g in gs:
+                if g is not None:
+                    self.lib.cs_graph_release(g)
+        for st in self.streams:
+            self.capi.check(self.lib.cs_stream_synchronize(st))
+            self.lib.cs_stream_destroy(st)
+
+
+def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None, streams=4):
     from compute import autotune
     from compute import tensors as tz
     lib = capi.lib
@@ -162,16 +233,40 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
 
     timer = Timer(capi, T.stream)
     steps = max(1, steps)
-    run_launches(steps % sets)  # records the tail graph outside the timed region
+    bytes_per_launch_c2 = 4 * n + 4 * 256 + 4 * n
+    # The timed region: exactly `steps` launches over the queue pool (the reference's
+    # execution model), after exactly `warmup` launches.
+    pool = QueuePool(T, capi, lambda st, i: capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr)),
+                     sets, streams)
+    pool.run(steps % sets)  # records the tail graphs outside the timed region
+    pool.run(warmup % sets)
     T.synchronize()
     if warmup > 0:
-        timer.measure(lambda: run_launches(warmup), 1)
-    ms = timer.measure(lambda: run_launches(steps), 1)
+        timer.measure(lambda: pool.run(warmup), 1)
+    ms = timer.measure(lambda: pool.run(steps), 1)
     launches = steps
     # the same kernel over a window of at least 4096 launches (about 28 ms), a figure that
-    # does not depend on how short the requested run is
+    # does not depend on how short the requested run is: over the pool, and one launch after
+    # another on one stream
     long_launches = max(4096, steps)
-    ms_long = timer.measure(lambda: run_launches(long_launches), 1)
+    ms_long = timer.measure(lambda: pool.run(long_launches), 1)
+    pool.release()
+    # other pool sizes over the same long window (the hardware has GPU_MAX_HW_QUEUES queues)
+    pool_sweep = {}
+    for k in (2, 8):
+        if k == streams:
+            continue
+        other = QueuePool(T, capi, lambda st, i: capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr)),
+                          sets, k)
+        other.run(0)
+        timer.measure(lambda: other.run(sets * 4), 1)
+        ms_k = timer.measure(lambda: other.run(long_launches), 1)
+        pool_sweep[k] = round(bytes_per_launch_c2 * long_launches / (ms_k * 1e-3) / 1e9, 1)
+        other.release()
+    run_launches(steps % sets)  # records the tail graph outside the timed region
+    T.synchronize()
+    ms_one = timer.measure(lambda: run_launches(steps), 1)
+    ms_one_long = timer.measure(lambda: run_launches(long_launches), 1)
     reps = max(1, long_launches // sets)
     # hot-cache figure: one input set, repeated
     g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
@@ -190,11 +285,14 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None):
                us_per_launch_events=us_event,
                long_window={"launches": long_launches, "us_per_launch": ms_long * 1e3 / long_launches,
                             "gbs": bytes_per * long_launches / (ms_long * 1e-3) / 1e9},
+               streams=streams, pool_sweep_gbs=pool_sweep,
+               one_stream={"launches": launches, "us_per_launch": ms_one * 1e3 / launches,
+                           "gbs": bytes_per * launches / (ms_one * 1e-3) / 1e9,
+                           "long_window_gbs": bytes_per * long_launches / (ms_one_long * 1e-3) / 1e9},
                copy_reference={"bytes": 8 * n, "us": ms_copy * 1e3 / (reps * sets),
                                "gbs": 8 * n * reps * sets / (ms_copy * 1e-3) / 1e9},
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
                gbs_hot_cache=bytes_per * reps * sets / (ms_hot * 1e-3) / 1e9, tuning=tuning)
-    res["overlapped"] = c2_overlapped(T, capi, f, ptrs, outs, bytes_per, long_launches)
     lib.cs_graph_release(g)
     lib.cs_graph_release(g_hot)
     for gt in tails.values():
@@ -275,61 +373,6 @@ def bench_hbm(T, capi, launches=1024):
         log("hbm probe %s per array: best read %.0f, write %.0f, copy %.0f GB/s" %
             (label, best["read"]["gbs"], best["write"]["gbs"], best["copy"]["gbs"]))
     return out
-
-
-def c2_overlapped(T, capi, f, ptrs, outs, bytes_per, launches, nstreams=4):
-    """The same launches with the 16 independent input sets spread over `nstreams` streams
-    (set i on stream i mod nstreams, each stream replaying a graph of its sets), so that one
-    launch's ramp-down overlaps the next one's start; timed from an event on the launch
-    stream that every stream waits for to events that the launch stream waits for. Reported
-    beside `value`, which runs the launches one after another."""
-    lib = capi.lib
-    sets = len(ptrs)
-    streams = []
-    for _ in range(nstreams):
-        st = ctypes.c_void_p()
-        capi.check(lib.cs_stream_create(T.device, ctypes.byref(st)))
-        streams.append(st)
-    graphs = []
-    for k, st in enumerate(streams):
-        mine = [i for i in range(sets) if i % nstreams == k]
-        for i in mine:
-            capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr))
-        capi.check(lib.cs_stream_synchronize(st))
-        capi.check(lib.cs_graph_begin_capture(st))
-        for i in mine:
-            capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr))
-        g = ctypes.c_void_p()
-        capi.check(lib.cs_graph_end_capture(st, ctypes.byref(g)))
-        graphs.append(g)
-    reps = max(1, launches // sets)
-
-    def run(r):
-        e0 = ctypes.c_void_p()
-        capi.check(lib.cs_event_record(T.stream, ctypes.byref(e0)))
-        for st in streams:
-            capi.check(lib.cs_stream_wait_event(st, e0))
-        for _ in range(r):
-            for st, g in zip(streams, graphs):
-                capi.check(lib.cs_graph_launch(st, g))
-        for st in streams:
-            e = ctypes.c_void_p()
-            capi.check(lib.cs_event_record(st, ctypes.byref(e)))
-            capi.check(lib.cs_stream_wait_event(T.stream, e))
-            lib.cs_event_release(e)
-        lib.cs_event_release(e0)
-
-    timer = Timer(capi, T.stream)
-    timer.measure(lambda: run(2), 1)
-    ms = timer.measure(lambda: run(reps), 1)
-    for g in graphs:
-        lib.cs_graph_release(g)
-    for st in streams:
-        capi.check(lib.cs_stream_synchronize(st))
-        lib.cs_stream_destroy(st)
-    n = reps * sets
-    return {"streams": nstreams, "launches": n, "us_per_launch": ms * 1e3 / n,
-            "gbs": bytes_per * n / (ms * 1e-3) / 1e9}
 
 
 def bench_c1(T, capi, steps, sets=16):
@@ -770,6 +813,55 @@ def bench_split(T, capi, n=64, seed=6001, out_dir=None):
     return res
 
 
+def profile_run(T, capi, what, count=512):
+    """Launches of one workload's kernels without graphs or extra streams, for a profiler to
+    time per dispatch: `c2` (the headline kernel over its 16 rotating sets, each launch
+    carrying its own dispatch events, whose average is printed for comparison), `c3` (the row
+    reduction), `c4` (one call of each GEMM variant, several times)."""
+    from compute import autotune
+    from compute import tensors as tz
+    lib = capi.lib
+    if what == "c2":
+        n = 64 * 256 * 256
+        sets = 16
+        xs = [T._alloc(n * 4) for _ in range(sets)]
+        bs = [T._alloc(256 * 4) for _ in range(sets)]
+        outs = [T._alloc(n * 4) for _ in range(sets)]
+        for i in range(sets):
+            capi.check(lib.cs_fill_normal(T.stream, xs[i].ptr, n, 2001 + 97 * i))
+            capi.check(lib.cs_fill_uniform(T.stream, bs[i].ptr, 256, 2002 + 97 * i, -0.5, 0.5))
+        X = tz.CachedTensor(T, (64, 256, 256), 0.0, xs[0].retain())
+        B = tz.CachedTensor(T, (256,), 0.0, bs[0].retain())
+        p = X.permute([1, 0, 2])
+        y = p + B.broadcast((256, 64, 256)) + p.translate([0, 0, 1])
+        out = T.Tensor.exp(-(y * y)) * T.Tensor.tanh(y)
+        cfg = {"vector_width": 4, "items_per_thread": 1, "block_size": 256, "nontemporal_store": 1}
+        f, _ = autotune.compile_with(T, out.closure, out.shape, cfg)
+        ptrs = [(ctypes.c_void_p * 2)(xs[i].ptr, bs[i].ptr) for i in range(sets)]
+        us = kernel_us_timed(capi, T.stream, f, ptrs, [o.ptr for o in outs], count)
+        log("c2 kernel: %.3f us per dispatch (its own dispatch events), %d launches, %.1f GB/s" %
+            (us, count, (8 * n + 1024) / (us * 1e-6) / 1e9))
+        lib.cs_fused_release(f)
+        for h in xs + bs + outs + [X, B]:
+            h.release()
+    elif what == "c3":
+        rows, cols = 16384, 4096
+        a = T._alloc(rows * cols * 4)
+        rs, rm, tot = T._alloc(rows * 4), T._alloc(rows * 4), T._alloc(8)
+        capi.check(lib.cs_fill_normal(T.stream, a.ptr, rows * cols, 3001))
+        for _ in range(count // 8):
+            capi.check(lib.cs_reduce_rows(T.stream, a.ptr, rows, cols, rs.ptr, rm.ptr, None, tot.ptr))
+        T.synchronize()
+        for h in (a, rs, rm, tot):
+            h.release()
+    elif what == "c4":
+        kinds = [("fp32", capi.GEMM_FP32), ("bf16", capi.GEMM_BF16), ("bf16x3", capi.GEMM_BF16X3),
+                 ("fp8", capi.GEMM_FP8)]
+        bench_gemm(T, capi, kinds, reps=10)
+    else:
+        raise SystemExit("--profile takes c2, c3 or c4")
+
+
 def cpu_model():
     """The host CPU's model name (/proc/cpuinfo) and its logical CPU count."""
     name = None
@@ -868,6 +960,8 @@ def main():
     ap.add_argument("--split-sweep", type=int, default=0,
                     help="time every fusion split of this many convolution-shaped joins and train the split model")
     ap.add_argument("--out-dir", default=None)
+    ap.add_argument("--streams", type=int, default=4,
+                    help="streams of the queue pool the headline's steps are spread over (1: one after another)")
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
     ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
@@ -877,6 +971,8 @@ def main():
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")
+    ap.add_argument("--profile", default=None,
+                    help="c2, c3 or c4: only that workload's kernels, launched without graphs (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every workload here runs on a single MI355X, "
@@ -896,14 +992,18 @@ def main():
         src.release()
         dst.release()
         log("copied %d MiB 8 times" % args.copy_mib)
+    if args.profile:
+        profile_run(T, capi, args.profile)
+        T.close()
+        return
     fixed = None
     if args.c2_config:
         v, u, b, nt = (int(x) for x in args.c2_config.split(","))
         fixed = {"vector_width": v, "items_per_thread": u, "block_size": b, "nontemporal_store": nt}
-    c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed)
-    log("c2: %.1f GB/s (%.2f us/launch), kernel %.2f us (dispatch events), hot cache %.1f GB/s, over %d streams "
-        "%.1f GB/s" % (c2["gbs"], c2["ms_per_launch"] * 1e3, c2["us_per_launch_events"], c2["gbs_hot_cache"],
-                       c2["overlapped"]["streams"], c2["overlapped"]["gbs"]))
+    c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed, streams=args.streams)
+    log("c2: %.1f GB/s over %d streams (%.2f us/launch), one stream %.1f GB/s, kernel %.2f us (dispatch events), "
+        "hot cache %.1f GB/s" % (c2["gbs"], c2["streams"], c2["ms_per_launch"] * 1e3, c2["one_stream"]["gbs"],
+                                 c2["us_per_launch_events"], c2["gbs_hot_cache"]))
     also = {}
     if "hbm" in only:
         also["hbm_ceiling"] = bench_hbm(T, capi)
@@ -971,10 +1071,12 @@ def main():
         "data": "synthetic (Wang-hash seeded)",
         "config": {"workload": "view-heavy fused chain: X[64,256,256] N(0,1), bias[256] U(-0.5,0.5); "
                                "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
-                               "exp(-y*y)*tanh(y); one generated kernel; 16 rotating input sets",
+                               "exp(-y*y)*tanh(y); one generated kernel per step; 16 rotating input sets; "
+                               "steps spread over a pool of %d streams" % c2["streams"],
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
-                   "long_window": c2["long_window"], "overlapped_streams": c2["overlapped"],
+                   "streams": c2["streams"], "long_window": c2["long_window"], "one_stream": c2["one_stream"],
+                   "other_pool_sizes_long_window_gbs": c2["pool_sweep_gbs"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch, the profiler's per-dispatch figure); `value` is the timed region's
