@@ -7,11 +7,10 @@ Headline (the `value` of the JSON line): the view-heavy fused chain
 as one generated kernel, in GB/s of algorithmic traffic (X read once, bias, out written:
 33,555,456 bytes per launch). One step = one launch over one input set; 16 input/output
 sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch streams
-from HBM. The K timed launches are spread over a pool of 4 streams (set i on stream i mod 4,
-as the reference's CommandQueuePool leases a queue per command), each replaying captured
-hipGraphs of its sets, and bracketed by HIP events on the launch stream that every stream
-waits for and waits on; the same launches one after another on one stream are reported
-beside it (config.one_stream).
+from HBM. The K timed launches are replayed from captured hipGraphs and bracketed by HIP
+events on the launch stream. Beside it: the same launches over a window of at least 4096,
+one after another and spread over a pool of 4 streams (set i on stream i mod 4, as the
+reference's CommandQueuePool leases a queue per command), whose launches overlap.
 
 The same line carries the other workloads under "also": a*b+c on [1024,1024] and its sum,
 the one-pass row sum/row max/total over [16384,4096], and C = A.B at 4096^3 per GEMM
@@ -234,21 +233,25 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None, streams=4):
     timer = Timer(capi, T.stream)
     steps = max(1, steps)
     bytes_per_launch_c2 = 4 * n + 4 * 256 + 4 * n
-    # The timed region: exactly `steps` launches over the queue pool (the reference's
-    # execution model), after exactly `warmup` launches.
-    pool = QueuePool(T, capi, lambda st, i: capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr)),
-                     sets, streams)
-    pool.run(steps % sets)  # records the tail graphs outside the timed region
-    pool.run(warmup % sets)
+    # The timed region: exactly `steps` launches one after another on the launch stream, after
+    # exactly `warmup` launches.
+    run_launches(steps % sets)  # records the tail graph outside the timed region
+    run_launches(warmup % sets)
     T.synchronize()
     if warmup > 0:
-        timer.measure(lambda: pool.run(warmup), 1)
-    ms = timer.measure(lambda: pool.run(steps), 1)
+        timer.measure(lambda: run_launches(warmup), 1)
+    ms = timer.measure(lambda: run_launches(steps), 1)
     launches = steps
     # the same kernel over a window of at least 4096 launches (about 28 ms), a figure that
-    # does not depend on how short the requested run is: over the pool, and one launch after
-    # another on one stream
+    # does not depend on how short the requested run is: one after another, and spread over
+    # a queue pool (the reference's execution model), whose fork and join cost a fixed
+    # ~0.3 ms that only a long run amortises
     long_launches = max(4096, steps)
+    ms_one_long = timer.measure(lambda: run_launches(long_launches), 1)
+    pool = QueuePool(T, capi, lambda st, i: capi.check(lib.cs_fused_launch(st, f, ptrs[i], 2, outs[i].ptr)),
+                     sets, streams)
+    pool.run(0)
+    timer.measure(lambda: pool.run(sets * 4), 1)
     ms_long = timer.measure(lambda: pool.run(long_launches), 1)
     pool.release()
     # other pool sizes over the same long window (the hardware has GPU_MAX_HW_QUEUES queues)
@@ -263,10 +266,6 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None, streams=4):
         ms_k = timer.measure(lambda: other.run(long_launches), 1)
         pool_sweep[k] = round(bytes_per_launch_c2 * long_launches / (ms_k * 1e-3) / 1e9, 1)
         other.release()
-    run_launches(steps % sets)  # records the tail graph outside the timed region
-    T.synchronize()
-    ms_one = timer.measure(lambda: run_launches(steps), 1)
-    ms_one_long = timer.measure(lambda: run_launches(long_launches), 1)
     reps = max(1, long_launches // sets)
     # hot-cache figure: one input set, repeated
     g_hot = capture(capi, T.stream, [lambda: launch(0)] * sets)
@@ -283,12 +282,12 @@ def bench_c2(T, capi, steps, warmup, sets=16, tune=True, fixed=None, streams=4):
     bytes_per = 4 * n + 4 * 256 + 4 * n
     res = dict(bytes_per_launch=bytes_per, launches=launches, ms=ms, ms_per_launch=ms / launches,
                us_per_launch_events=us_event,
-               long_window={"launches": long_launches, "us_per_launch": ms_long * 1e3 / long_launches,
-                            "gbs": bytes_per * long_launches / (ms_long * 1e-3) / 1e9},
-               streams=streams, pool_sweep_gbs=pool_sweep,
-               one_stream={"launches": launches, "us_per_launch": ms_one * 1e3 / launches,
-                           "gbs": bytes_per * launches / (ms_one * 1e-3) / 1e9,
-                           "long_window_gbs": bytes_per * long_launches / (ms_one_long * 1e-3) / 1e9},
+               long_window={"launches": long_launches, "us_per_launch": ms_one_long * 1e3 / long_launches,
+                            "gbs": bytes_per * long_launches / (ms_one_long * 1e-3) / 1e9},
+               stream_pool={"streams": streams, "launches": long_launches,
+                            "us_per_launch": ms_long * 1e3 / long_launches,
+                            "gbs": bytes_per * long_launches / (ms_long * 1e-3) / 1e9,
+                            "other_pool_sizes_gbs": pool_sweep},
                copy_reference={"bytes": 8 * n, "us": ms_copy * 1e3 / (reps * sets),
                                "gbs": 8 * n * reps * sets / (ms_copy * 1e-3) / 1e9},
                gbs=bytes_per * launches / (ms * 1e-3) / 1e9,
@@ -961,7 +960,7 @@ def main():
                     help="time every fusion split of this many convolution-shaped joins and train the split model")
     ap.add_argument("--out-dir", default=None)
     ap.add_argument("--streams", type=int, default=4,
-                    help="streams of the queue pool the headline's steps are spread over (1: one after another)")
+                    help="streams of the queue pool the long-window figure is spread over")
     ap.add_argument("--c2-config", default=None,
                     help="V,U,BLOCK,NT: launch configuration of the headline kernel instead of autotuning it")
     ap.add_argument("--gemm", default="fp32,bf16,bf16x3,fp8", help="GEMM variants to run for c4")
@@ -1001,9 +1000,10 @@ def main():
         v, u, b, nt = (int(x) for x in args.c2_config.split(","))
         fixed = {"vector_width": v, "items_per_thread": u, "block_size": b, "nontemporal_store": nt}
     c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed, streams=args.streams)
-    log("c2: %.1f GB/s over %d streams (%.2f us/launch), one stream %.1f GB/s, kernel %.2f us (dispatch events), "
-        "hot cache %.1f GB/s" % (c2["gbs"], c2["streams"], c2["ms_per_launch"] * 1e3, c2["one_stream"]["gbs"],
-                                 c2["us_per_launch_events"], c2["gbs_hot_cache"]))
+    log("c2: %.1f GB/s (%.2f us/launch), long window %.1f GB/s, over %d streams %.1f GB/s, kernel %.2f us "
+        "(dispatch events), hot cache %.1f GB/s" % (c2["gbs"], c2["ms_per_launch"] * 1e3, c2["long_window"]["gbs"],
+                                                    c2["stream_pool"]["streams"], c2["stream_pool"]["gbs"],
+                                                    c2["us_per_launch_events"], c2["gbs_hot_cache"]))
     also = {}
     if "hbm" in only:
         also["hbm_ceiling"] = bench_hbm(T, capi)
@@ -1071,12 +1071,10 @@ def main():
         "data": "synthetic (Wang-hash seeded)",
         "config": {"workload": "view-heavy fused chain: X[64,256,256] N(0,1), bias[256] U(-0.5,0.5); "
                                "p=X.permute(1,0,2); y=p+bias.broadcast([256,64,256])+p.translate([0,0,1]); "
-                               "exp(-y*y)*tanh(y); one generated kernel per step; 16 rotating input sets; "
-                               "steps spread over a pool of %d streams" % c2["streams"],
+                               "exp(-y*y)*tanh(y); one generated kernel per step; 16 rotating input sets",
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
-                   "streams": c2["streams"], "long_window": c2["long_window"], "one_stream": c2["one_stream"],
-                   "other_pool_sizes_long_window_gbs": c2["pool_sweep_gbs"],
+                   "long_window": c2["long_window"], "stream_pool": c2["stream_pool"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch, the profiler's per-dispatch figure); `value` is the timed region's

@@ -67,3 +67,36 @@ def test_shipped_cost_model_on_held_out_expressions():
         assert chosen <= 1.10 * best, (len(cand), chosen / best)
         if cand is b256:
             assert chosen <= untuned, (chosen, untuned)
+
+
+def test_shipped_split_model_on_held_out_joins():
+    """The shipped fusion-split model against the exhaustive autotuner's own timings (the saved
+    sweep of every cut of 256 convolution-shaped joins, seed 6001; the model saw the first 192):
+    on the 64 held-out joins its cuts cost at most 10% more than the best cut of each, and no
+    more than 2% above never cutting (the reference's one kernel). The network is evaluated by
+    the FP64 oracle's forward pass on the saved parameters, with the saved margin."""
+    import gzip
+    import json
+    import os
+
+    from compute import Tensors, autotune, costmodel, workloads
+    data = os.path.join(os.path.dirname(costmodel.__file__), "data")
+    sweep = json.load(gzip.open(os.path.join(data, "split_timings.json.gz")))
+    model = np.load(os.path.join(data, "splitmodel.npz"))
+    widths = [int(w) for w in model["widths"]]
+    n = sweep["joins"]
+    times = {(k, g): us for k, g, us in sweep["times"]}
+    joins = workloads.conv_joins(Tensors(offline=True), n, sweep["seed"])
+    ntrain = (3 * n) // 4
+    best = chosen = never = 0.0
+    for k in range(ntrain, n):
+        cands = autotune.split_candidates(joins[k].parts)
+        X = np.stack([costmodel.split_features(joins[k].tensor, g) for g in cands])
+        pred = mlp_ref.forward(model["params"].astype(np.float64), widths, (X - model["mean"]) / model["std"])[-1]
+        pick = costmodel.pick_groups(cands, pred[:, 0], float(model["margin"][0]))
+        best += min(times[(k, g)] for g in cands)
+        chosen += times[(k, pick)]
+        never += times[(k, 1)]
+    print("held-out joins: exhaustive %.1f us, model %.1f us, one kernel %.1f us" % (best, chosen, never))
+    assert chosen <= 1.10 * best
+    assert chosen <= 1.02 * never
