@@ -1095,6 +1095,13 @@ cs_status cs_graph_end_capture(cs_stream *stream, cs_graph **graph) {
       (void)hipGraphDestroy(h);
       CS_HIP(r);
     }
+    // Uploaded now, so that the first launch costs what later ones do.
+    r = hipGraphUpload(x, stream->stream);
+    if (r != hipSuccess) {
+      (void)hipGraphExecDestroy(x);
+      (void)hipGraphDestroy(h);
+      CS_HIP(r);
+    }
     auto *g = new cs_graph();
     g->graph = h;
     g->exec = x;
