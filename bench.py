@@ -843,15 +843,15 @@ def bench_split(T, capi, n=64, seed=6001, out_dir=None):
 
 def profile_run(T, capi, what, count=512):
     """Launches of one workload's kernels without graphs or extra streams, for a profiler to
-    time per dispatch: `c2` (the headline kernel over its 16 rotating sets, each launch
-    carrying its own dispatch events, whose average is printed for comparison), `c3` (the row
-    reduction), `c4` (one call of each GEMM variant, several times)."""
+    time per dispatch: `c2` (the headline kernel over the headline's rotating sets, plain
+    launches one after another on the launch stream), `c3` (the row reduction), `c4` (one
+    call of each GEMM variant, several times)."""
     from compute import autotune
     from compute import tensors as tz
     lib = capi.lib
     if what == "c2":
         n = 64 * 256 * 256
-        sets = 16
+        sets = HEADLINE_SETS
         xs = [T._alloc(n * 4) for _ in range(sets)]
         bs = [T._alloc(256 * 4) for _ in range(sets)]
         outs = [T._alloc(n * 4) for _ in range(sets)]
@@ -866,9 +866,11 @@ def profile_run(T, capi, what, count=512):
         cfg = {"vector_width": 4, "items_per_thread": 1, "block_size": 256, "nontemporal_store": 1}
         f, _ = autotune.compile_with(T, out.closure, out.shape, cfg)
         ptrs = [(ctypes.c_void_p * 2)(xs[i].ptr, bs[i].ptr) for i in range(sets)]
-        us = kernel_us_timed(capi, T.stream, f, ptrs, [o.ptr for o in outs], count)
-        log("c2 kernel: %.3f us per dispatch (its own dispatch events), %d launches, %.1f GB/s" %
-            (us, count, (8 * n + 1024) / (us * 1e-6) / 1e9))
+        # plain launches, one after another on the launch stream, as a graph replay issues them
+        for k in range(count):
+            capi.check(lib.cs_fused_launch(T.stream, f, ptrs[k % sets], 2, outs[k % sets].ptr))
+        T.synchronize()
+        log("c2 kernel: %d plain launches" % count)
         lib.cs_fused_release(f)
         for h in xs + bs + outs + [X, B]:
             h.release()
