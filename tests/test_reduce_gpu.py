@@ -119,6 +119,67 @@ def test_total_of_many_rows(gpu_tensors):
 
 
 @pytest.mark.parametrize("kind", ["sum", "max", "rows"])
+def test_combine_reads_every_new_partial(gpu_tensors, kind):
+    """The combine adds the partials that this call's workgroups wrote. Two inputs of the same
+    shape alternate over 200 back-to-back calls with no host wait between them, so each call's
+    partials overwrite the previous call's in the same scratch: a combine that read one stale
+    partial would mix the two inputs. Then the
+    same pair of launches captured in a graph, replayed five times with other reductions on
+    the stream between replays. Every result must carry its own input's bits."""
+    T = gpu_tensors
+    rows, cols = 4096, 1024
+    xs = [inputs.heavy_tails(inputs.normal(rows * cols, 501 + k), 503 + k).reshape(rows, cols) for k in range(2)]
+    bufs = [upload(T, x) for x in xs]
+    launches = 200
+    outs = alloc(T, launches)
+    outs64 = alloc(T, launches, np.float64)
+    f4, f8 = ctypes.sizeof(ctypes.c_float), ctypes.sizeof(ctypes.c_double)
+
+    def launch(i, k, slot):
+        o, o64 = outs.ptr + slot * f4, outs64.ptr + slot * f8
+        if kind == "sum":
+            capi.check(capi.lib.cs_reduce_sum(T.stream, bufs[k].ptr, rows * cols, o, o64))
+        elif kind == "max":
+            capi.check(capi.lib.cs_reduce_max(T.stream, bufs[k].ptr, rows * cols, o))
+        else:
+            capi.check(capi.lib.cs_reduce_rows(T.stream, bufs[k].ptr, rows, cols, None, None, o, o64))
+
+    for i in range(launches):
+        launch(i, i % 2, i)
+    got = download(T, outs, launches)
+    got64 = download(T, outs64, launches, np.float64)
+    for k in range(2):
+        mine = got[k::2]
+        assert np.array_equal(mine.view(np.uint32), np.full(len(mine), mine[0], np.float32).view(np.uint32)), k
+        if kind == "max":
+            assert mine[0] == xs[k].max()
+        else:
+            assert np.all(got64[k::2] == got64[k])
+            exact = xs[k].astype(np.float64).sum()
+            assert abs(got64[k] - exact) <= 1e-12 * np.abs(xs[k]).astype(np.float64).sum()
+    assert got[0] != got[1]
+    first = got[:2].copy()
+    # the same two launches in a graph, replayed with eager reductions between the replays
+    capi.check(capi.lib.cs_graph_begin_capture(T.stream))
+    launch(0, 0, 0)
+    launch(1, 1, 1)
+    g = ctypes.c_void_p()
+    capi.check(capi.lib.cs_graph_end_capture(T.stream, ctypes.byref(g)))
+    for r in range(5):
+        capi.check(capi.lib.cs_graph_launch(T.stream, g))
+        launch(2 + 2 * r, 1, 2 + 2 * r)
+        launch(3 + 2 * r, 0, 3 + 2 * r)
+        replay = download(T, outs, 2)
+        assert np.array_equal(replay.view(np.uint32), first.view(np.uint32)), r
+    capi.lib.cs_graph_release(g)
+    eager = download(T, outs, 12)
+    assert np.array_equal(eager[2:12:2].view(np.uint32), np.full(5, first[1], np.float32).view(np.uint32))
+    assert np.array_equal(eager[3:12:2].view(np.uint32), np.full(5, first[0], np.float32).view(np.uint32))
+    for h in bufs + [outs, outs64]:
+        h.release()
+
+
+@pytest.mark.parametrize("kind", ["sum", "max", "rows"])
 def test_combine_is_deterministic(gpu_tensors, kind):
     """Block partials are combined in FP64 in a fixed order: 300 back-to-back launches (each
     followed by a one-block reduction on the same stream) must all give the first launch's

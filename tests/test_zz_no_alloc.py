@@ -1,5 +1,5 @@
-"""cs_reduce_rows_noalloc, the row reduction that allocates nothing (its counter and row
-partials sit in a fixed device array of the library), against cs_reduce_rows: the same bits
+"""cs_reduce_rows_noalloc, the row reduction that allocates nothing (the FP64 partials of its
+total sit in a fixed device array of the library), against cs_reduce_rows: the same bits
 for row sums, row maxima and both totals, on two calls in a row, the second with a smaller
 input; rows beyond its limit are refused when a total is asked for."""
 import ctypes
