@@ -118,8 +118,9 @@ class StreamSpread(object):
 
     ROUNDS = 16  # rounds of the sets in one replay of the full graphs
 
-    def __init__(self, T, capi, launch_on, sets, k):
+    def __init__(self, T, capi, launch_on, sets, k, prime=False):
         self.T, self.capi, self.lib = T, capi, capi.lib
+        self.prime = prime  # run each stream's sets once before capturing its graphs
         self.launch_on, self.sets, self.k = launch_on, sets, k
         self.streams, self.made = [T.stream], []
         for _ in range(k - 1):
@@ -136,6 +137,11 @@ class StreamSpread(object):
         gs = []
         for j, st in enumerate(self.streams):
             mine = [i % self.sets for i in range(count) if (i % self.sets) % self.k == j]
+            if mine and self.prime:
+                # sizes the stream's scratch (reductions) outside the capture
+                for i in sorted(set(mine)):
+                    self.launch_on(st, i)
+                self.capi.check(self.lib.cs_stream_synchronize(st))
             gs.append(capture(self.capi, st, [lambda i=i, st=st: self.launch_on(st, i) for i in mine])
                       if mine else None)
         return gs
@@ -403,7 +409,7 @@ def bench_hbm(T, capi, launches=1024, streams=3):
     return out
 
 
-def bench_c1(T, capi, steps, sets=16):
+def bench_c1(T, capi, steps, sets=30, streams=3):
     lib = capi.lib
     n = 1024 * 1024
     a = [[T._alloc(n * 4) for _ in range(3)] for _ in range(sets)]
@@ -446,7 +452,23 @@ def bench_c1(T, capi, steps, sets=16):
     L = reps * sets
     res = {"fma_gbs": 16 * n * L / (ms_fma * 1e-3) / 1e9, "fma_us": ms_fma * 1e3 / L, "fma_config": results[0][0],
            "sum_gbs": (4 * n + 4) * L / (ms_sum * 1e-3) / 1e9, "sum_us": ms_sum * 1e3 / L,
-           "fused_fma_sum_us": ms_fsum * 1e3 / L, "fused_fma_sum_gbs": (12 * n + 4) * L / (ms_fsum * 1e-3) / 1e9}
+           "fused_fma_sum_us": ms_fsum * 1e3 / L, "fused_fma_sum_gbs": (12 * n + 4) * L / (ms_fsum * 1e-3) / 1e9,
+           "sets": sets}
+    # the same three calls spread over `streams` streams, as the headline is, over 4096 calls
+    spread = {}
+    for name, nbytes, fn in (
+            ("fma", 16 * n, lambda st, i: capi.check(lib.cs_fused_launch(st, f, ptrs[i], 3, outs[i].ptr))),
+            ("sum", 4 * n + 4, lambda st, i: capi.check(lib.cs_reduce_sum(st, outs[i].ptr, n, sums[i].ptr, None))),
+            ("fused_fma_sum", 12 * n + 4,
+             lambda st, i: capi.check(lib.cs_fused_sum(st, fs, ptrs[i], 3, sums[i].ptr, None)))):
+        sp = StreamSpread(T, capi, fn, sets, streams, prime=True)
+        sp.run(sets)
+        ms = sp.run(4096)
+        sp.release()
+        spread[name + "_us"] = round(ms * 1e3 / 4096, 3)
+        spread[name + "_gbs"] = round(nbytes * 4096 / (ms * 1e-3) / 1e9, 1)
+    spread["streams"] = streams
+    res["over_streams"] = spread
     for g in (g1, g2, g3):
         lib.cs_graph_release(g)
     lib.cs_fused_release(f)
@@ -458,7 +480,7 @@ def bench_c1(T, capi, steps, sets=16):
     return res
 
 
-def bench_c3(T, capi, steps, sets=2):
+def bench_c3(T, capi, steps, sets=3, streams=3):
     lib = capi.lib
     rows, cols = 16384, 4096
     n = rows * cols
@@ -484,6 +506,15 @@ def bench_c3(T, capi, steps, sets=2):
     bytes_per = 4 * n + 4 * rows * 2 + 8
     res = {"rows_gbs": bytes_per * L / (ms * 1e-3) / 1e9, "rows_us": ms * 1e3 / L, "bytes_per_launch": bytes_per}
     lib.cs_graph_release(g)
+    # the same calls spread over `streams` streams (each with its own scratch), 384 calls
+    sp = StreamSpread(T, capi, lambda st, i: capi.check(lib.cs_reduce_rows(st, ins[i].ptr, rows, cols, rs[i].ptr,
+                                                                            rm[i].ptr, None, tot[i].ptr)),
+                      sets, streams, prime=True)
+    sp.run(sets)
+    ms_sp = sp.run(384)
+    sp.release()
+    res["over_streams"] = {"streams": streams, "calls": 384, "rows_us": round(ms_sp * 1e3 / 384, 3),
+                           "rows_gbs": round(bytes_per * 384 / (ms_sp * 1e-3) / 1e9, 1)}
     # column sums of the same matrix (an outer-axis reduction: `split(0).reduce(_ + _)`)
     cs = [T._alloc(cols * 4) for _ in range(sets)]
     for i in range(sets):  # sizes the stream's scratch before capture
@@ -1040,10 +1071,10 @@ def main():
     if "hbm" in only:
         also["hbm_ceiling"] = bench_hbm(T, capi, streams=args.streams)
     if "c1" in only:
-        also["c1_fma_sum"] = bench_c1(T, capi, max(2048, args.steps))
+        also["c1_fma_sum"] = bench_c1(T, capi, max(2048, args.steps), streams=args.streams)
         log("c1: %s" % also["c1_fma_sum"])
     if "c3" in only:
-        also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16))
+        also["c3_rows_sum_max_total"] = bench_c3(T, capi, max(64, args.steps // 16), streams=args.streams)
         log("c3: %s" % also["c3_rows_sum_max_total"])
     if "c4" in only:
         kinds = {"fp32": capi.GEMM_FP32, "bf16": capi.GEMM_BF16, "bf16x3": capi.GEMM_BF16X3, "fp8": capi.GEMM_FP8}
