@@ -532,11 +532,12 @@ def bench_c3(T, capi, steps, sets=3, streams=3):
     return res
 
 
-def bench_gemm(T, capi, variants, reps=10):
+def bench_gemm(T, capi, variants, reps=10, streams=2):
     lib = capi.lib
     n = 4096
     from oracle import fp64, inputs
     ba, bb, bc = T._alloc(n * n * 4), T._alloc(n * n * 4), T._alloc(n * n * 4)
+    bc2 = T._alloc(n * n * 4)  # the second stream's output when calls are spread
     rows = np.arange(3, n, 16)  # 256 rows of C against FP64 per operand set
     sets = {}
     for scaled in (False, True):
@@ -569,9 +570,23 @@ def bench_gemm(T, capi, variants, reps=10):
         out[name] = {"tflops": tflops, "ms": ms / reps, "normalized_error": errs["uniform"],
                      "normalized_error_row_scaled": errs["row_scaled"],
                      "mfma_util": tflops * (3 if name == "bf16x3" else 1) / PEAK_TFLOPS[name]}
+        if streams > 1:
+            # independent calls (the same operands, an output per stream) spread over
+            # `streams` streams, each with its own operand scratch: one call's operand packing
+            # overlaps another's main kernel
+            outs_c = [bc, bc2]
+            sp = StreamSpread(T, capi, lambda st, i: capi.check(lib.cs_gemm(st, v, n, n, n, ba.ptr, n, bb.ptr, n,
+                                                                             outs_c[i].ptr, n)),
+                              2, streams, prime=True)
+            sp.run(2)
+            calls = 2 * reps
+            ms_sp = sp.run(calls)
+            sp.release()
+            out[name]["over_streams"] = {"streams": streams, "calls": calls,
+                                         "tflops": 2.0 * n ** 3 * calls / (ms_sp * 1e-3) / 1e12}
         log("gemm %s: %.1f TFLOP/s, error %.3g (row-scaled set %.3g)" % (name, tflops, errs["uniform"],
                                                                          errs["row_scaled"]))
-    for h in (ba, bb, bc):
+    for h in (ba, bb, bc, bc2):
         h.release()
     return out
 
@@ -918,7 +933,7 @@ def profile_run(T, capi, what, count=512):
     elif what == "c4":
         kinds = [("fp32", capi.GEMM_FP32), ("bf16", capi.GEMM_BF16), ("bf16x3", capi.GEMM_BF16X3),
                  ("fp8", capi.GEMM_FP8)]
-        bench_gemm(T, capi, kinds, reps=10)
+        bench_gemm(T, capi, kinds, reps=10, streams=1)
     else:
         raise SystemExit("--profile takes c2, c3 or c4")
 
@@ -1082,6 +1097,11 @@ def main():
         g = also["c4_gemm_4096"]
         if "bf16x3" in g and "fp32" in g:
             g["bf16x3_over_fp32"] = g["bf16x3"]["tflops"] / g["fp32"]["tflops"]
+            if "over_streams" in g["bf16x3"] and "over_streams" in g["fp32"]:
+                g["bf16x3_over_fp32_over_streams"] = (g["bf16x3"]["over_streams"]["tflops"] /
+                                                      g["fp32"]["over_streams"]["tflops"])
+        log("gemm over streams: %s" % {k: round(x["over_streams"]["tflops"], 1) for k, x in g.items()
+                                       if isinstance(x, dict) and "over_streams" in x})
     if args.split_sweep:
         also["split_sweep"] = bench_split(T, capi, args.split_sweep, out_dir=args.out_dir)
         log("split sweep: %s" % also["split_sweep"])
