@@ -5,10 +5,10 @@ Headline (the `value` of the JSON line): the view-heavy fused chain
     p = X.permute(1,0,2); y = p + bias.broadcast([256,64,256]) + p.translate([0,0,1])
     out = exp(-(y*y)) * tanh(y)
 as one generated kernel, in GB/s of algorithmic traffic (X read once, bias, out written:
-33,555,456 bytes per launch). One step = one launch over one input set; 15 input/output
-sets rotate (480 MiB in total, nearly twice the 256 MB Infinity Cache) so every launch
-streams from HBM. The K timed launches are spread over 3 streams (launch i on stream
-i mod 3, as the reference's CommandQueuePool leases a queue per command), each stream
+33,555,456 bytes per launch). One step = one launch over one input set; 16 input/output
+sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch streams
+from HBM. The K timed launches are spread over 3 streams (launch i on stream
+(i mod 16) mod 3, as the reference's CommandQueuePool leases a queue per command), each stream
 replaying captured hipGraphs of its share, and timed from the earliest start event to the
 latest end event of the streams. Beside it: the same K launches on one stream, and a window
 of at least 4096 launches over 1 to 4 streams.
@@ -36,10 +36,10 @@ sys.path.insert(0, ROOT)
 METRIC = "fused-expr HBM GB/s (% of 8 TB/s); GEMM TFLOP/s at err≤1e-5 vs FP64; vs plain"
 HBM_PEAK_GBS = 8000.0
 PEAK_TFLOPS = {"fp32": 157.3, "bf16": 2500.0, "bf16x3": 2500.0, "fp8": 5000.0}
-# Input/output sets the headline rotates through: 15 x 32 MiB = 480 MiB, nearly twice the
-# 256 MB Infinity Cache, and within the GPU's TLB reach (past 16 sets the headline kernel,
-# whose permuted read touches every 2 MiB page of X from the first wave, slows by a fifth).
-HEADLINE_SETS = 15
+# Input/output sets the headline rotates through: 16 x 32 MiB = 512 MiB, twice the 256 MB
+# Infinity Cache, and within the GPU's TLB reach (past 16 sets the headline kernel, whose
+# permuted read touches every 2 MiB page of X from the first wave, slows by a fifth).
+HEADLINE_SETS = 16
 
 
 def log(msg):
@@ -409,7 +409,7 @@ def bench_hbm(T, capi, launches=1024, streams=3):
     return out
 
 
-def bench_c1(T, capi, steps, sets=30, streams=3):
+def bench_c1(T, capi, steps, sets=33, streams=3):
     lib = capi.lib
     n = 1024 * 1024
     a = [[T._alloc(n * 4) for _ in range(3)] for _ in range(sets)]
