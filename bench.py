@@ -690,6 +690,19 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     g_ch = graph_each(chosen)
     us_each = g_ch.time_us(5)
     g_ch.release()
+    # the same per-expression launches spread over three streams (expression i on stream
+    # i mod 3, each stream replaying a graph of its expressions), as the reference's queue
+    # pool runs independent materialisations concurrently
+    def launch_one_on(st, i):
+        c = chosen[i]
+        f = fused[(c["vector_width"], c["items_per_thread"])][i]
+        capi.check(capi.lib.cs_fused_launch_block(st, f, eset._ptrs[i], len(eset.leaf_bufs[i]),
+                                                  eset.out_bufs[i].ptr, c["block_size"]))
+    sp = StreamSpread(T, capi, launch_one_on, len(exprs), 3)
+    sp.run(len(exprs))
+    us_each_streams = min(sp.run(len(exprs)) for _ in range(5)) * 1e3
+    sp.release()
+    res["per_expression_over_3_streams_us"] = us_each_streams
     eset.clear_outputs()
     sel = [fused[(c["vector_width"], c["items_per_thread"])][i] for i, c in enumerate(chosen)]
     for i, f in enumerate(sel):
@@ -754,8 +767,11 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     res["per_expression_gbs"] = total_bytes / (us_each * 1e-6) / 1e9
     res["batched_gbs"] = total_bytes / (us_batch * 1e-6) / 1e9
     res["batched_over_per_expression"] = us_each / us_batch
-    log("c5: per-expression %.1f us (default configs %.1f us), batched %.1f us -> %.2fx, %.0f GB/s, identical=%s" %
-        (us_each, us_def, us_batch, res["batched_over_per_expression"], res["batched_gbs"], res["batch_bit_identical"]))
+    res["batched_over_per_expression_over_3_streams"] = us_each_streams / us_batch
+    log("c5: per-expression %.1f us (default configs %.1f us, over 3 streams %.1f us), batched %.1f us -> %.2fx "
+        "(%.2fx against 3 streams), %.0f GB/s, identical=%s" %
+        (us_each, us_def, us_each_streams, us_batch, res["batched_over_per_expression"],
+         res["batched_over_per_expression_over_3_streams"], res["batched_gbs"], res["batch_bit_identical"]))
     for fs in fused.values():
         for f in fs:
             capi.lib.cs_fused_release(f)
@@ -934,8 +950,27 @@ def profile_run(T, capi, what, count=512):
         kinds = [("fp32", capi.GEMM_FP32), ("bf16", capi.GEMM_BF16), ("bf16x3", capi.GEMM_BF16X3),
                  ("fp8", capi.GEMM_FP8)]
         bench_gemm(T, capi, kinds, reps=10, streams=1)
+    elif what == "c5":
+        from compute import batch as bm
+        from compute import workloads
+        exprs = workloads.uneven_dags(T, 512, 5001)
+        eset = bm.ExpressionSet(T, exprs)
+        fs = bm.compile_many(T, exprs, [{"vector_width": 2, "items_per_thread": 2, "block_size": 256,
+                                         "nontemporal_store": 1}] * len(exprs))
+        b = bm.Batch(eset, fs, block=512)
+        for _ in range(3):
+            b.launch()
+        T.synchronize()
+        for i, f in enumerate(fs):
+            eset.launch(i, f, 256)
+        T.synchronize()
+        log("c5: %d expressions, %d bytes; 3 batch launches, then each expression once" % (len(exprs), eset.bytes))
+        b.release()
+        for f in fs:
+            lib.cs_fused_release(f)
+        eset.release()
     else:
-        raise SystemExit("--profile takes c2, c3 or c4")
+        raise SystemExit("--profile takes c2, c3, c4 or c5")
 
 
 def cpu_model():
@@ -1049,7 +1084,7 @@ def main():
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")
     ap.add_argument("--profile", default=None,
-                    help="c2, c3 or c4: only that workload's kernels, launched without graphs (for rocprofv3)")
+                    help="c2, c3, c4 or c5: only that workload's kernels, launched without graphs (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every workload here runs on a single MI355X, "

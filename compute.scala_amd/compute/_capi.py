@@ -41,6 +41,7 @@ class CsFusedOptions(ctypes.Structure):
         ("items_per_thread", ctypes.c_int32),
         ("block_size", ctypes.c_int32),
         ("nontemporal_store", ctypes.c_int32),
+        ("launch_order", ctypes.c_int32),
     ]
 
 
@@ -63,6 +64,7 @@ CHECK_EXP_BOUNDED = 114  # cs_math_check: exp where its argument cannot exceed 8
 OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD, OP_MIN, OP_MAX = 20, 21, 22, 23, 24, 25, 26
 
 MATH_PRECISE, MATH_FAST, MATH_ACCURATE = 0, 1, 2
+ORDER_AUTO, ORDER_LINEAR = 0, 1
 PROBE_READ, PROBE_WRITE, PROBE_COPY = 0, 1, 2
 PROBE_STORE_PLAIN, PROBE_STORE_NT, PROBE_STORE_NT_WRITE_THROUGH = 0, 1, 2
 

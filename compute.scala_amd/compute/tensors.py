@@ -869,8 +869,10 @@ class Tensors(object):
 
     def __init__(self, ordinal=0, math_mode=capi.MATH_PRECISE, vector_width=0, items_per_thread=0,
                  block_size=0, offline=False, axis_reductions=True, fused_sums=True, gemm_idioms="large",
-                 join_split=1):
-        self.options = capi.CsFusedOptions(math_mode, vector_width, items_per_thread, block_size, 0)
+                 join_split=1, launch_order=capi.ORDER_AUTO):
+        # launch_order: ORDER_AUTO tiles a kernel's outputs across the dimension along which a
+        # transposed leaf's rows run (same bits, fewer bytes fetched); ORDER_LINEAR never does
+        self.options = capi.CsFusedOptions(math_mode, vector_width, items_per_thread, block_size, 0, launch_order)
         # The reference's two matrix-multiply idioms (README.md:301-362) as one FP32 MFMA GEMM:
         # "large" when the unrolled kernel would hold at least MM1_MIN_TERMS products per
         # output (matrixMultiply1) or the [i,j,k] product at least MM2_MIN_ELEMENTS elements

@@ -290,6 +290,7 @@ std::string float_literal(float v) {
 struct Options {
   int math_mode = CS_MATH_PRECISE;
   int V = 0, U = 0, block = 0, nt = 0;
+  int order = CS_ORDER_AUTO;
   bool sum = false;  // the kernel adds its single output into FP64 block partials instead of storing it
 };
 
@@ -301,7 +302,9 @@ Options resolve_options(const cs_fused_options *o, const Dag &dag) {
     r.U = o->items_per_thread;
     r.block = o->block_size;
     r.nt = o->nontemporal_store;
+    r.order = o->launch_order;
   }
+  if (r.order != CS_ORDER_AUTO && r.order != CS_ORDER_LINEAR) fail(CS_INVALID_VALUE, "unknown launch order");
   if (r.math_mode != CS_MATH_PRECISE && r.math_mode != CS_MATH_FAST && r.math_mode != CS_MATH_ACCURATE)
     fail(CS_INVALID_VALUE, "unknown math mode");
   int64_t last = dag.out_shape.empty() ? 1 : dag.out_shape.back();
@@ -357,6 +360,7 @@ std::string canonical_key(const Dag &dag, const Options &o) {
   put(k, o.block);
   put(k, o.nt);
   if (o.sum) put(k, int32_t(1));  // appended only in sum mode: element-wise keys are unchanged
+  if (o.order != CS_ORDER_AUTO) put(k, int32_t(100 + o.order));  // likewise for the default order
   return k;
 }
 
@@ -448,24 +452,114 @@ class Emitter {
   }
 
   // tile: the index of the block's slice of blockDim.x * U vectors.
+  // Launch order. A leaf read through a view whose source rows run along an outer output
+  // dimension d (the source's innermost coordinate moves with d alone, by +-1: a transpose)
+  // is read one element per source row by a wave that walks the output's innermost
+  // dimension, so each load takes a few bytes of a line and the rest must wait in the caches.
+  // Then workgroups take tiles of R positions of d by C vectors of the dimensions after d
+  // (C fastest): R consecutive elements of each source row are read together, and the output
+  // and every other leaf are still written and read in runs of C vectors (whole rows when
+  // they are short). d is the dimension with the most such leaf elements. R is d's extent up
+  // to 32, else the size in 16..32 that pads it least; C likewise up to 16, in 8..16. The
+  // tiles cover the padded extents, whose extra positions are idle lanes; the order stays
+  // linear when R < 4 or the padding exceeds 1/8 of the output. Which thread computes which
+  // element changes, not how: results are the same bits.
+  struct Order {
+    bool tiled = false;
+    int64_t Sd = 0, Bv = 0, R = 0, C = 0, Sdp = 0, Bvp = 0, iter = 0;
+  };
+  static int64_t least_padding(int64_t extent, int64_t lo, int64_t hi) {
+    if (extent <= hi) return extent;
+    int64_t best = hi, pad = (extent + hi - 1) / hi * hi;
+    for (int64_t c = hi - 1; c >= lo; --c) {
+      const int64_t p = (extent + c - 1) / c * c;
+      if (p < pad) pad = p, best = c;
+    }
+    return best;
+  }
+  Order choose_order(int64_t vectors) const {
+    Order ord;
+    const int r = static_cast<int>(dag_.out_shape.size());
+    if (o_.order == CS_ORDER_LINEAR || o_.sum || r < 2) return ord;
+    const int64_t V = o_.V;
+    if (dag_.out_shape[r - 1] % V != 0) return ord;
+    std::vector<double> weight(static_cast<size_t>(r), 0.0);
+    for (const cs_node &n : dag_.nodes) {
+      if (n.op != CS_OP_EXTRACT) continue;
+      ArrayRef ar = resolve_array(dag_, n.a);
+      if (ar.is_fill || !ar.has_view || ar.out_rank != r) continue;
+      const cs_node &p = dag_.nodes[ar.param];
+      if (p.rank < 2 || p.shape[p.rank - 1] < 2) continue;
+      const int cols = r + 1, y = p.rank - 1;
+      int drive = -1, count = 0;
+      bool unit = true;
+      for (int x = 0; x < r; ++x) {
+        const double m = ar.matrix[static_cast<size_t>(y) * cols + x];
+        if (m == 0.0) continue;
+        ++count;
+        drive = x;
+        unit = unit && (m == 1.0 || m == -1.0);
+      }
+      if (count != 1 || !unit || drive == r - 1) continue;
+      double elements = 1.0;
+      for (int i = 0; i < p.rank; ++i) elements *= p.shape[i];
+      weight[static_cast<size_t>(drive)] += elements;
+    }
+    int d = -1;
+    for (int x = 0; x < r - 1; ++x)
+      if (weight[static_cast<size_t>(x)] > 0.0 && (d < 0 || weight[static_cast<size_t>(x)] > weight[static_cast<size_t>(d)]))
+        d = x;
+    if (d < 0) return ord;
+    int64_t B = 1;
+    for (int x = d + 1; x < r; ++x) B *= dag_.out_shape[static_cast<size_t>(x)];
+    const int64_t Sd = dag_.out_shape[static_cast<size_t>(d)], Bv = B / V;
+    const int64_t R = least_padding(Sd, 16, 32), C = least_padding(Bv, 8, 16);
+    if (R < 4) return ord;
+    const int64_t Sdp = (Sd + R - 1) / R * R, Bvp = (Bv + C - 1) / C * C;
+    const int64_t iter = vectors / (Sd * Bv) * Sdp * Bvp;
+    if (iter * 8 > vectors * 9 || iter >= (int64_t(1) << 31)) return ord;
+    ord.tiled = true;
+    ord.Sd = Sd, ord.Bv = Bv, ord.R = R, ord.C = C, ord.Sdp = Sdp, ord.Bvp = Bvp, ord.iter = iter;
+    return ord;
+  }
+
   void body(std::ostringstream &s, const std::string &tile, int64_t *vectors_out) {
     const int r = static_cast<int>(dag_.out_shape.size());
     const int V = o_.V, U = o_.U;
     int64_t total = 1;
     for (auto d : dag_.out_shape) total *= d;
     const int64_t vectors = total / V;
-    *vectors_out = vectors;
     const int nout = static_cast<int>(dag_.outputs.size());
     uniform_.clear();
     fill_of_.clear();
     chunks_.clear();
+    const Order ord = choose_order(vectors);
+    // the launch covers the tiles' padded extents when tiled
+    *vectors_out = ord.tiled ? ord.iter : vectors;
     s << "  const unsigned vbase = (" << tile << ") * (" << bdim_ << " * " << U << "u) + threadIdx.x;\n";
     // phase 0: vector index, validity and coordinates of element 0 of every vector
     s << "  unsigned q[" << U << "]; bool live[" << U << "];\n";
     for (int d = 0; d < r; ++d) s << "  int c" << d << "[" << U << "];\n";
     s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
     s << "    q[u] = vbase + u * " << bdim_ << ";\n";
-    s << "    live[u] = q[u] < " << vectors << "u;\n";
+    if (!ord.tiled) {
+      s << "    live[u] = q[u] < " << vectors << "u;\n";
+    } else {
+      // iteration index -> vector index: tiles of R positions of dimension d by C vectors of
+      // the dimensions after it, C fastest, over the padded extents; tiles in row-major order
+      // of (outer, d, inner); a padding position is an idle lane
+      const int64_t T = ord.R * ord.C, tiles_b = ord.Bvp / ord.C, tiles_d = ord.Sdp / ord.R;
+      s << "    live[u] = q[u] < " << ord.iter << "u;\n";
+      s << "    {\n";
+      s << "      const unsigned t = q[u] % " << T << "u, tl = q[u] / " << T << "u;\n";
+      s << "      const unsigned bt = tl % " << tiles_b << "u, rt = tl / " << tiles_b << "u;\n";
+      s << "      const unsigned dt = rt % " << tiles_d << "u, ot = rt / " << tiles_d << "u;\n";
+      s << "      const unsigned dd = dt * " << ord.R << "u + t / " << ord.C << "u, bb = bt * " << ord.C << "u + t % "
+        << ord.C << "u;\n";
+      s << "      live[u] = live[u] && dd < " << ord.Sd << "u && bb < " << ord.Bv << "u;\n";
+      s << "      q[u] = live[u] ? (ot * " << ord.Sd << "u + dd) * " << ord.Bv << "u + bb : " << vectors << "u;\n";
+      s << "    }\n";
+    }
     s << "    unsigned rest = q[u] * " << V << "u;\n";
     for (int d = r - 1; d >= 0; --d) {
       if (d > 0)
@@ -924,7 +1018,7 @@ struct cs_fused {
   hipFunction_t function = nullptr;
   int nleaves = 0;
   int64_t out_elements = 0;  // coordinates x tuple width
-  int64_t vectors = 0;       // number of V-wide output vectors
+  int64_t vectors = 0;       // V-wide vectors the launch covers (the output, padded when tiled)
   uint32_t block = 256;
   uint32_t grid = 0;
   int max_block = 1024;      // the kernel's launch bound

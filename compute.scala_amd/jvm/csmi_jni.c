@@ -217,13 +217,15 @@ static int read_ints(JNIEnv *env, jintArray a, int32_t *dst, jsize cap, jsize *n
   return 1;
 }
 
-/* options: {math_mode, vector_width, items_per_thread, block_size, nontemporal_store} */
+/* options: {math_mode, vector_width, items_per_thread, block_size, nontemporal_store
+ * [, launch_order]}; a missing launch_order is CS_ORDER_AUTO */
 static int read_options(JNIEnv *env, jintArray options, cs_fused_options *o) {
-  int32_t v[5] = {0, 0, 0, 0, 0};
+  int32_t v[6] = {0, 0, 0, 0, 0, CS_ORDER_AUTO};
   jsize n;
-  if (!read_ints(env, options, v, 5, &n) || n != 5) return 0;
+  if (!read_ints(env, options, v, 6, &n) || (n != 5 && n != 6)) return 0;
   o->math_mode = v[0], o->vector_width = v[1], o->items_per_thread = v[2], o->block_size = v[3];
   o->nontemporal_store = v[4];
+  o->launch_order = v[5];
   return 1;
 }
 

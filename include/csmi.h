@@ -226,7 +226,14 @@ typedef struct cs_fused_options {
   int32_t items_per_thread;   /* vectors per thread: 1..8; 0 = auto */
   int32_t block_size;         /* threads per workgroup: 64..1024; 0 = auto */
   int32_t nontemporal_store;  /* 1 = stream the output past the caches */
+  int32_t launch_order;       /* cs_launch_order: which outputs a workgroup computes together */
 } cs_fused_options;
+
+/* CS_ORDER_AUTO: when a leaf is read through a view whose source rows run along an outer
+ * output dimension (a transpose), workgroups take 2-D tiles across that dimension, so the
+ * leaf's rows are read in whole cache lines; otherwise, and for fused sums, the linear order.
+ * CS_ORDER_LINEAR: consecutive vectors of the output, always. Results are the same bits. */
+enum { CS_ORDER_AUTO = 0, CS_ORDER_LINEAR = 1 };
 
 /* Compiles (or fetches from the structural cache) the kernel that evaluates
  * outputs[0..noutputs) at every coordinate of out_shape. noutputs > 1 writes an

@@ -59,6 +59,37 @@ def test_c2_views_fold_into_one_kernel(offline_tensors):
     compiles(src)
 
 
+def test_launch_order_tiles_across_a_transpose(offline_tensors):
+    """A leaf read transposed (its rows run along output dimension 0) makes the kernel take
+    tiles across that dimension: 32 positions by 16 vectors for [96, 64], the whole extents
+    for [7, 9]. C2's permute keeps X's rows along the innermost dimension and stays linear,
+    as do a transpose whose rows are too short (2 elements) and the linear launch order."""
+    from compute import Tensors
+    T = offline_tensors
+    x = T.Tensor(np.zeros((64, 96), np.float32))
+    y = T.Tensor(np.zeros((96, 64), np.float32))
+    r = x.permute([1, 0]) * T.Tensor.fill(2.0, (96, 64)) + y
+    src = T.kernel_source(r.closure, r.shape)
+    assert "% 512u" in src and "dt * 32u" in src  # R = 32 positions of dimension 0, C = 16 vectors
+    compiles(src)
+    assert "dt * " not in T.kernel_source(c2_expr(T).closure, (256, 64, 256))
+    small = T.Tensor(np.zeros((9, 7), np.float32)).permute([1, 0]) + T.Tensor.fill(1.0, (7, 9))
+    src = T.kernel_source(small.closure, small.shape)
+    assert "dt * 7u" in src  # R = 7: the whole extent
+    compiles(src)
+    short = T.Tensor(np.zeros((64, 2), np.float32)).permute([1, 0]) + T.Tensor.fill(1.0, (2, 64))
+    assert "dt * " not in T.kernel_source(short.closure, short.shape)
+    lin = Tensors(offline=True, launch_order=capi.ORDER_LINEAR)
+    x2 = lin.Tensor(np.zeros((64, 96), np.float32))
+    r2 = x2.permute([1, 0]) * lin.Tensor.fill(2.0, (96, 64))
+    assert "dt * " not in lin.kernel_source(r2.closure, r2.shape)
+    x3 = T.Tensor(np.zeros((4, 64, 96), np.float32))
+    r3 = x3.permute([2, 0, 1]) + T.Tensor.fill(1.0, (96, 4, 64))  # X's rows run along dimension 0
+    src3 = T.kernel_source(r3.closure, r3.shape)
+    assert "dt * 32u" in src3
+    compiles(src3)
+
+
 @pytest.mark.parametrize("shape", [(), (7,), (3, 5), (2, 3, 4, 5), (2, 1, 3, 1, 2, 2)])
 def test_shapes_compile(offline_tensors, shape):
     T = offline_tensors
