@@ -95,7 +95,8 @@ object Csmi {
                           blocking: Boolean): Int
   @native def bufferRead(stream: Long, buffer: Long, offset: Long, host: ByteBuffer, bytes: Long): Int
 
-  /** options: Array(mathMode, vectorWidth, itemsPerThread, blockSize, nontemporalStore). */
+  /** options: Array(mathMode, vectorWidth, itemsPerThread, blockSize, nontemporalStore[, launchOrder]);
+    * launchOrder 0 (the default) tiles across a transposed leaf, 1 keeps the linear order. */
   @native def fusedCompile(device: Long, nodes: ByteBuffer, nnodes: Int, outputs: Array[Int], shape: Array[Int],
                            options: Array[Int], out: Array[Long]): Int
   @native def fusedCompileSum(device: Long, nodes: ByteBuffer, nnodes: Int, outputs: Array[Int],
