@@ -261,6 +261,8 @@ def bench_c2(T, capi, steps, warmup, sets=HEADLINE_SETS, tune=True, fixed=None, 
         spread.run(warmup)
     ms = spread.run(steps)
     launches = steps
+    # ten more runs of the same K launches (after the timed one): how much the figure moves
+    reruns = sorted(bytes_per_launch_c2 * steps / (spread.run(steps) * 1e-3) / 1e9 for _ in range(10))
     # The same kernel over a window of at least 4096 launches (about 20-30 ms), a figure that
     # does not depend on how short the requested run is, spread over 1 to 4 streams.
     long_launches = max(4096, steps)
@@ -298,6 +300,8 @@ def bench_c2(T, capi, steps, warmup, sets=HEADLINE_SETS, tune=True, fixed=None, 
                us_per_launch_events=us_event,
                one_stream={"launches": steps, "us_per_launch": ms_one * 1e3 / steps,
                            "gbs": bytes_per * steps / (ms_one * 1e-3) / 1e9},
+               reruns={"runs": len(reruns), "min_gbs": round(reruns[0], 1),
+                       "median_gbs": round((reruns[4] + reruns[5]) / 2, 1), "max_gbs": round(reruns[-1], 1)},
                long_window={"launches": long_launches, "streams": streams,
                             "us_per_launch": ms_long * 1e3 / long_launches,
                             "gbs": bytes_per * long_launches / (ms_long * 1e-3) / 1e9,
@@ -1113,10 +1117,11 @@ def main():
         v, u, b, nt = (int(x) for x in args.c2_config.split(","))
         fixed = {"vector_width": v, "items_per_thread": u, "block_size": b, "nontemporal_store": nt}
     c2 = bench_c2(T, capi, args.steps, args.warmup, fixed=fixed, streams=args.streams)
-    log("c2: %.1f GB/s (%.2f us/launch) over %d streams, one stream %.1f GB/s, long window %.1f GB/s "
-        "(by streams %s), kernel %.2f us (dispatch events), hot cache %.1f GB/s" %
-        (c2["gbs"], c2["ms_per_launch"] * 1e3, args.streams, c2["one_stream"]["gbs"], c2["long_window"]["gbs"],
-         c2["long_window"]["gbs_by_streams"], c2["us_per_launch_events"], c2["gbs_hot_cache"]))
+    log("c2: %.1f GB/s (%.2f us/launch) over %d streams (10 reruns %s), one stream %.1f GB/s, long window "
+        "%.1f GB/s (by streams %s), kernel %.2f us (dispatch events), hot cache %.1f GB/s" %
+        (c2["gbs"], c2["ms_per_launch"] * 1e3, args.streams, c2["reruns"], c2["one_stream"]["gbs"],
+         c2["long_window"]["gbs"], c2["long_window"]["gbs_by_streams"], c2["us_per_launch_events"],
+         c2["gbs_hot_cache"]))
     also = {}
     if "hbm" in only:
         also["hbm_ceiling"] = bench_hbm(T, capi, streams=args.streams)
@@ -1196,7 +1201,8 @@ def main():
                                "steps spread over %d streams" % (HEADLINE_SETS, args.streams),
                    "bytes_per_step": c2["bytes_per_launch"], "hot_cache_gbs": round(c2["gbs_hot_cache"], 2),
                    "copy_reference": c2["copy_reference"], "pcie_inclusive": c2["pcie_inclusive"],
-                   "streams": args.streams, "one_stream": c2["one_stream"], "long_window": c2["long_window"],
+                   "streams": args.streams, "one_stream": c2["one_stream"], "reruns": c2["reruns"],
+                   "long_window": c2["long_window"],
                    "launch_config": c2["tuning"]},
         # achieved: algorithmic bytes over the kernel's own average duration (events stamped by
         # each dispatch run alone, the profiler's per-dispatch figure). `value` is the timed
