@@ -839,7 +839,7 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
 
 def bench_split(T, capi, n=64, seed=6001, out_dir=None):
     """Fusion splits of convolution-shaped joins: the exhaustive autotuner times every cut of
-    each of n joins (1, 2, 4 or 8 kernels plus the gathering kernel), the split cost model is
+    each of n joins (1, 2, 4 or 8 kernels plus the gathering kernel; 20 graph replays each), the split cost model is
     trained on the first 3/4 of the joins (by the device MLP kernels) and its picks on the
     rest are compared with the exhaustive best and with never splitting. Saves the timings and
     the model beside the package's data when asked (`--split-sweep`)."""
@@ -857,7 +857,7 @@ def bench_split(T, capi, n=64, seed=6001, out_dir=None):
     times = {}
     for k, j in enumerate(joins):
         for g in autotune.split_candidates(j.parts):
-            times[(k, g)] = autotune.time_join(T, j.tensor, ptrs, g)
+            times[(k, g)] = autotune.time_join(T, j.tensor, ptrs, g, reps=20)
         if k % 16 == 15:
             log("split sweep: %d/%d joins timed (%.0f s)" % (k + 1, n, time.time() - t0))
     for b in bufs:

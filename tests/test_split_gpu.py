@@ -51,7 +51,7 @@ def test_split_model_on_new_joins():
                 ptrs[id(t)] = buf.ptr
         best = chosen = never = 0.0
         for j in joins:
-            times = {g: autotune.time_join(T, j.tensor, ptrs, g) for g in autotune.split_candidates(j.parts)}
+            times = {g: autotune.time_join(T, j.tensor, ptrs, g, reps=20) for g in autotune.split_candidates(j.parts)}
             pick = model.choose_groups(j.tensor)
             best += min(times.values())
             chosen += times[pick]
