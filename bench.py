@@ -8,9 +8,10 @@ as one generated kernel, in GB/s of algorithmic traffic (X read once, bias, out 
 33,555,456 bytes per launch). One step = one launch over one input set; 16 input/output
 sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch streams
 from HBM. The K timed launches are spread over 3 streams (launch i on stream
-(i mod 16) mod 3, as the reference's CommandQueuePool leases a queue per command), each stream
-replaying captured hipGraphs of its share, and timed from the earliest start event to the
-latest end event of the streams. Beside it: the same K launches on one stream, and a window
+(i mod 16) mod 3, as the reference's CommandQueuePool leases a queue per command and its JMH
+benchmarks run independent iterations from every core at once, benchmarks.scala:31-32, 56),
+each stream replaying captured hipGraphs of its share, and timed from the earliest start
+event to the latest end event of the streams. Beside it: the same K launches on one stream, and a window
 of at least 4096 launches over 1 to 4 streams.
 
 The same line carries the other workloads under "also": a*b+c on [1024,1024] and its sum,
