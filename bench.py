@@ -784,7 +784,7 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     return res
 
 
-def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
+def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1, items_per_thread=0, block_size=256):
     """The convolution idiom of the reference's benchmark at its largest setting
     (benchmarks.scala:454-548, 612-631: input [128,32,32,8], 3x3 kernel, 8 filters, one
     layer): a join of 8 per-filter sums of 72 products of translated input channels and
@@ -793,7 +793,8 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
     Reported separately (not a headline workload)."""
     from compute import Tensors
     from compute import tensors as tz, trees, workloads
-    T = Tensors(0, math_mode=T_outer.options.math_mode, vector_width=vector_width, block_size=256)
+    T = Tensors(0, math_mode=T_outer.options.math_mode, vector_width=vector_width,
+                items_per_thread=items_per_thread, block_size=block_size)
     lib = capi.lib
     B, H, W, D, F = 128, 32, 32, 8, 8
     n_in, n_w, n_out = B * H * W * D, 3 * 3 * D * F, B * H * W * F
@@ -827,7 +828,8 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1):
     flop = 2 * B * H * W * F * 9 * D + B * H * W * F  # products and adds per output, plus the bias
     res = {"us_per_forward": us, "gflops": flop / (us * 1e-6) / 1e9,
            "gbs": 4 * (n_in + n_w + F + n_out) / (us * 1e-6) / 1e9, "compile_s": compile_s,
-           "config": "input [128,32,32,8], weight [3,3,8,8], bias [8], one layer, V=%d" % vector_width}
+           "config": "input [128,32,32,8], weight [3,3,8,8], bias [8], one layer, V=%d, U=%d, %d threads" %
+                     (vector_width, T.options.items_per_thread, block_size)}
     lib.cs_graph_release(g)
     lib.cs_fused_release(f)
     for t in (X, Wt, Bt):
