@@ -569,6 +569,9 @@ class Emitter {
     }
     s << "  }\n";
     body_ = &s;
+    staged_.clear();
+    if (gq_.empty()) stage_constants(s);
+    plan_row_groups();
     // phase 1: gathers
     for (size_t i = 0; i < dag_.nodes.size(); ++i)
       if (dag_.nodes[i].op == CS_OP_EXTRACT) emit_gather(static_cast<int>(i));
@@ -725,39 +728,22 @@ class Emitter {
 
   // A leaf read at the current output coordinates (OpenCLKernelBuilder.scala:348-411 for
   // views, 435-455 for a bare parameter).
-  void emit_gather(int id) {
-    std::ostringstream &s = *body_;
-    const int V = o_.V, U = o_.U;
+  struct Row {
+    bool empty = true;        // all-zero row: index 0, no bounds check
+    bool fractional = false;  // double arithmetic, truncated by (int)
+    std::string expr;         // index of element 0 of the vector
+    int64_t inner_step = 0;   // integer coefficient of the innermost output coordinate
+    bool upper_only = false;  // bare parameter: only idx < dim is checked
+    int64_t shift = 0;        // integral constant term
+    bool others_by4 = true;   // integral coefficients of the other coordinates, all multiples of 4
+    std::string unshifted;    // expr without the integral constant term
+  };
+
+  // One Row per source dimension of the leaf a gather reads: the index expression of element
+  // 0 of the vector and its properties.
+  std::vector<Row> gather_rows(const ArrayRef &ar, const cs_node &p) const {
     const int r = static_cast<int>(dag_.out_shape.size());
-    const cs_node &n = dag_.nodes[id];
-    ArrayRef ar = resolve_array(dag_, n.a);
-    std::string name = "t" + std::to_string(id);
-    if (ar.is_fill) {
-      // extract(fill(x)) == x; x is computed in the arithmetic phase, so alias it there
-      fill_of_[id] = ar.fill_value;
-      return;
-    }
-    const cs_node &p = dag_.nodes[ar.param];
     const int src_rank = p.rank;
-    const std::string ptr = "p" + std::to_string(p.param);
-    const std::string pad = float_literal(p.value);
-    if (src_rank == 0) {
-      // `*x`: a scalar source is read without bounds (OpenCLKernelBuilder.scala:351-356).
-      uniform_.insert(id);
-      s << "  float " << name << "[" << U << "];\n";
-      s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) " << name << "[u] = " << ptr << "[0];\n";
-      return;
-    }
-    struct Row {
-      bool empty = true;        // all-zero row: index 0, no bounds check
-      bool fractional = false;  // double arithmetic, truncated by (int)
-      std::string expr;         // index of element 0 of the vector
-      int64_t inner_step = 0;   // integer coefficient of the innermost output coordinate
-      bool upper_only = false;  // bare parameter: only idx < dim is checked
-      int64_t shift = 0;        // integral constant term
-      bool others_by4 = true;   // integral coefficients of the other coordinates, all multiples of 4
-      std::string unshifted;    // expr without the integral constant term
-    };
     std::vector<Row> rows(static_cast<size_t>(src_rank));
     if (!ar.has_view) {
       if (src_rank > r)
@@ -804,6 +790,226 @@ class Emitter {
         row.unshifted = u;
       }
     }
+    return rows;
+  }
+
+  // The flat offset of a gather whose every index is an integral constant inside the leaf (an
+  // element picked out by split, then broadcast), with its leaf in *leaf; -1 for any other
+  // gather (one that moves with the coordinates, truncates a fraction, or reads the padding).
+  int64_t constant_offset(int id, int *leaf) const {
+    ArrayRef ar = resolve_array(dag_, dag_.nodes[id].a);
+    if (ar.is_fill) return -1;
+    const cs_node &p = dag_.nodes[ar.param];
+    const int r = static_cast<int>(dag_.out_shape.size()), cols = r + 1;
+    if (p.rank > 0 && (!ar.has_view || ar.out_rank != r)) return -1;
+    int64_t off = 0, stride = 1;
+    for (int y = p.rank - 1; y >= 0; --y) {
+      for (int x = 0; x < r; ++x)
+        if (ar.matrix[static_cast<size_t>(y) * cols + x] != 0.0) return -1;
+      const double m = ar.matrix[static_cast<size_t>(y) * cols + r];
+      int64_t i = 0;
+      if (m != 0.0) {
+        const std::string lit = decimal_format(m);
+        if (lit.find('.') != std::string::npos) return -1;
+        i = std::stoll(lit);
+      }
+      if (i < 0 || i >= p.shape[y]) return -1;
+      off += i * stride;
+      stride *= p.shape[y];
+    }
+    *leaf = p.param;
+    return off;
+  }
+
+  // Constant tables in LDS. A body that reads many single elements of small leaves (the
+  // weights and biases of the convolution idiom: 576 + 8 scalars, each broadcast over the
+  // output) would otherwise issue one per-lane global load per scalar in every thread. Each
+  // such leaf's referenced range is copied once per workgroup into csmi_k, and the gathers
+  // read it there (one broadcast LDS read each). The values are the leaf's own, so results
+  // are the same bits. Kernels only (a batch's bodies would add up their tables), and only
+  // with at least kMinStaged such gathers, in at most kMaxStaged floats.
+  static constexpr int64_t kMinStaged = 32, kMaxStaged = 4096;
+  void stage_constants(std::ostringstream &s) {
+    std::map<int, std::pair<int64_t, int64_t>> range;  // leaf -> [lo, hi]
+    std::map<int, int> uses;
+    for (size_t i = 0; i < dag_.nodes.size(); ++i) {
+      if (dag_.nodes[i].op != CS_OP_EXTRACT) continue;
+      int leaf = -1;
+      const int64_t off = constant_offset(static_cast<int>(i), &leaf);
+      if (off < 0) continue;
+      auto it = range.find(leaf);
+      if (it == range.end())
+        range[leaf] = {off, off};
+      else
+        it->second = {std::min(it->second.first, off), std::max(it->second.second, off)};
+      ++uses[leaf];
+    }
+    int64_t total = 0, count = 0;
+    std::vector<std::pair<int, std::pair<int64_t, int64_t>>> chosen;
+    for (const auto &kv : range) {
+      const int64_t len = kv.second.second - kv.second.first + 1;
+      if (total + len > kMaxStaged) continue;
+      chosen.push_back({kv.first, {kv.second.first, total}});
+      total += len;
+      count += uses[kv.first];
+    }
+    if (count < kMinStaged) return;
+    s << "  __shared__ float csmi_k[" << total << "];\n";
+    for (const auto &c : chosen) {
+      const int64_t len = range[c.first].second - range[c.first].first + 1;
+      s << "  for (unsigned i = threadIdx.x; i < " << len << "u; i += blockDim.x) csmi_k[" << c.second.second
+        << " + i] = p" << c.first << "[" << c.second.first << " + i];\n";
+      staged_[c.first] = c.second;
+    }
+    s << "  __syncthreads();\n";
+  }
+
+  // Rows read together. With one element per vector (V = 1), gathers of one leaf whose
+  // indices agree in every source dimension but the last, where each has an in-range
+  // constant (the channels of a pixel, each split out and read through the same view, as in
+  // the convolution idiom), read one source row: they share one bounds check and row address,
+  // and the constants that fall two or more into one aligned 4-element chunk of the row load
+  // it as one 16-byte access (the row length a multiple of 4). Each value is the element, or
+  // the padding, that its own gather reads.
+  struct RowGroup {
+    std::vector<std::pair<int, int64_t>> members;  // (node, constant), in node order
+    bool emitted = false;
+  };
+  void plan_row_groups() {
+    row_groups_.clear();
+    in_group_.clear();
+    if (o_.V != 1) return;
+    const int r = static_cast<int>(dag_.out_shape.size());
+    std::map<std::string, RowGroup> cand;
+    for (size_t i = 0; i < dag_.nodes.size(); ++i) {
+      if (dag_.nodes[i].op != CS_OP_EXTRACT) continue;
+      ArrayRef ar = resolve_array(dag_, dag_.nodes[i].a);
+      if (ar.is_fill || !ar.has_view || ar.out_rank != r) continue;
+      const cs_node &p = dag_.nodes[ar.param];
+      if (p.rank < 2) continue;
+      int leaf = -1;
+      if (constant_offset(static_cast<int>(i), &leaf) >= 0 && staged_.count(leaf)) continue;
+      const std::vector<Row> rows = gather_rows(ar, p);
+      const Row &last = rows[static_cast<size_t>(p.rank - 1)];
+      if (!last.empty && (last.fractional || !last.unshifted.empty())) continue;
+      const int64_t k = last.empty ? 0 : last.shift;
+      if (k < 0 || k >= p.shape[p.rank - 1]) continue;
+      std::string key = "p" + std::to_string(p.param);
+      bool ok = true;
+      for (int y = 0; y + 1 < p.rank; ++y) {
+        ok = ok && !rows[static_cast<size_t>(y)].fractional;
+        key += "|" + (rows[static_cast<size_t>(y)].empty ? std::string("_") : rows[static_cast<size_t>(y)].expr);
+      }
+      if (ok) cand[key].members.push_back({static_cast<int>(i), k});
+    }
+    for (auto &kv : cand) {
+      if (kv.second.members.size() < 2) continue;
+      for (const auto &m : kv.second.members) in_group_[m.first] = kv.first;
+      row_groups_[kv.first] = kv.second;
+    }
+  }
+
+  // Emits every gather of a row group, at its first member.
+  void emit_row_group(RowGroup &grp, const ArrayRef &ar) {
+    std::ostringstream &s = *body_;
+    const int U = o_.U;
+    const cs_node &p = dag_.nodes[ar.param];
+    const std::string ptr = "p" + std::to_string(p.param), pad = float_literal(p.value);
+    const std::string g = "r" + std::to_string(grp.members.front().first);
+    const std::vector<Row> rows = gather_rows(ar, p);
+    const int L = p.rank - 1;
+    int64_t elements = 1;
+    for (int y = 0; y < p.rank; ++y) elements *= p.shape[y];
+    std::map<int64_t, int> per_chunk;
+    for (const auto &m : grp.members) ++per_chunk[m.second / 4];
+    const bool chunks = p.shape[L] % 4 == 0 && elements > 0;
+    for (const auto &m : grp.members) {
+      s << "  float t" << m.first << "[" << U << "];\n";
+      uniform_.insert(m.first);
+    }
+    s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) {\n";
+    std::vector<std::string> conds{"live[u]"};
+    std::string off;
+    int64_t stride = p.shape[L];
+    std::vector<std::string> terms;
+    for (int y = L - 1; y >= 0; --y) {
+      const Row &row = rows[static_cast<size_t>(y)];
+      const std::string ix = g + "_i" + std::to_string(y);
+      if (row.empty) {
+        stride *= p.shape[y];
+        continue;
+      }
+      s << "      const int " << ix << " = (int)(" << row.expr << ");\n";
+      conds.push_back(ix + " >= 0");
+      conds.push_back(ix + " < " + std::to_string(p.shape[y]));
+      terms.push_back(ix + " * " + std::to_string(stride));
+      stride *= p.shape[y];
+    }
+    for (auto it = terms.rbegin(); it != terms.rend(); ++it) off += (off.empty() ? "" : " + ") + *it;
+    std::string cond;
+    for (size_t c = 0; c < conds.size(); ++c) cond += (c ? " && " : "") + conds[c];
+    s << "      const bool " << g << "_ok = " << cond << ";\n";
+    s << "      const " << gq_ << "float* " << g << "_p = " << ptr << (off.empty() ? "" : " + (" + off + ")") << ";\n";
+    std::set<int64_t> loaded;
+    for (const auto &m : grp.members) {
+      const int64_t q = m.second / 4;
+      if (chunks && per_chunk[q] >= 2) {
+        const std::string w = g + "_c" + std::to_string(q);
+        if (loaded.insert(q).second)
+          s << "      const csmi_f4 " << w << " = *(const " << gq_ << "csmi_f4*)(" << g << "_ok ? " << g << "_p + "
+            << 4 * q << " : " << ptr << ");\n";
+        s << "      t" << m.first << "[u] = " << g << "_ok ? " << w << "[" << m.second % 4 << "] : " << pad << ";\n";
+      } else {
+        s << "      t" << m.first << "[u] = " << g << "_ok ? " << g << "_p[" << m.second << "] : " << pad << ";\n";
+      }
+    }
+    s << "  }\n";
+    grp.emitted = true;
+  }
+
+  void emit_gather(int id) {
+    std::ostringstream &s = *body_;
+    const int V = o_.V, U = o_.U;
+    const int r = static_cast<int>(dag_.out_shape.size());
+    const cs_node &n = dag_.nodes[id];
+    ArrayRef ar = resolve_array(dag_, n.a);
+    std::string name = "t" + std::to_string(id);
+    if (ar.is_fill) {
+      // extract(fill(x)) == x; x is computed in the arithmetic phase, so alias it there
+      fill_of_[id] = ar.fill_value;
+      return;
+    }
+    if (!staged_.empty()) {
+      int leaf = -1;
+      const int64_t off = constant_offset(id, &leaf);
+      auto it = off < 0 ? staged_.end() : staged_.find(leaf);
+      if (it != staged_.end()) {
+        uniform_.insert(id);
+        s << "  float " << name << "[" << U << "];\n";
+        s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) " << name << "[u] = live[u] ? csmi_k["
+          << it->second.second + off - it->second.first << "] : " << float_literal(dag_.nodes[ar.param].value)
+          << ";\n";
+        return;
+      }
+    }
+    auto member = in_group_.find(id);
+    if (member != in_group_.end()) {
+      RowGroup &grp = row_groups_[member->second];
+      if (!grp.emitted) emit_row_group(grp, ar);
+      return;
+    }
+    const cs_node &p = dag_.nodes[ar.param];
+    const int src_rank = p.rank;
+    const std::string ptr = "p" + std::to_string(p.param);
+    const std::string pad = float_literal(p.value);
+    if (src_rank == 0) {
+      // `*x`: a scalar source is read without bounds (OpenCLKernelBuilder.scala:351-356).
+      uniform_.insert(id);
+      s << "  float " << name << "[" << U << "];\n";
+      s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) " << name << "[u] = " << ptr << "[0];\n";
+      return;
+    }
+    std::vector<Row> rows = gather_rows(ar, p);
     std::vector<int64_t> stride(static_cast<size_t>(src_rank), 1);
     for (int y = src_rank - 2; y >= 0; --y) stride[y] = stride[y + 1] * p.shape[y + 1];
 
@@ -981,6 +1187,9 @@ class Emitter {
   std::set<int> uniform_;
   std::map<int, int> fill_of_;
   std::map<std::string, std::string> chunks_;  // aligned source chunks already loaded, by position
+  std::map<int, std::pair<int64_t, int64_t>> staged_;  // leaf -> (first staged offset, its slot in csmi_k)
+  std::map<std::string, RowGroup> row_groups_;          // rows read together, by leaf and leading indices
+  std::map<int, std::string> in_group_;                 // gather node -> its row group
 };
 
 // FAST mirrors -cl-unsafe-math-optimizations (OpenCL.scala:1131-1135): reassociation,
