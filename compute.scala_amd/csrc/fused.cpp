@@ -385,14 +385,33 @@ class Emitter {
 
   // The whole kernel source: a stand-alone kernel over blockIdx.x.
   std::string generate(int64_t *vectors_out, const std::string &name, bool with_prelude = true) {
-    std::ostringstream s;
+    std::ostringstream b, s;
+    decls_.clear();
+    prefix_ = name;
+    body(b, "blockIdx.x", vectors_out);
+    prefix_.clear();
     header(s, with_prelude);
-    s << "extern \"C\" __global__ __launch_bounds__(" << launch_bound(dag_, o_) << ") void " << name << "(";
+    s << decls_;
+    s << "extern \"C\" __global__ __launch_bounds__(" << launch_bound(dag_, o_) << ")" << waves_hint(*vectors_out)
+      << " void " << name << "(";
     params(s);
-    s << ") {\n";
-    body(s, "blockIdx.x", vectors_out);
-    s << "}\n";
+    s << ") {\n" << b.str() << "}\n";
     return s.str();
+  }
+
+  // A large body (compiled for its own workgroup size) whose grid cannot fill the chip gets
+  // the register budget of the occupancy that grid reaches, at least 2 waves per SIMD: with
+  // 256 CUs of 4 SIMDs (MI355X), a grid of at most 2048 waves never has more than 2 waves on
+  // a SIMD, and registers reserved for more only keep the compiler from holding loads in
+  // flight (the convolution idiom's forward pass: 12.5 us without the hint, 9.9 us with it).
+  static constexpr int64_t kComputeUnits = 256;
+  std::string waves_hint(int64_t vectors) const {
+    if (dag_.nodes.size() <= kLargeBody || o_.sum) return "";
+    const int64_t block = o_.block, per_group = (block + 63) / 64;
+    const int64_t groups = (vectors + block * o_.U - 1) / (block * o_.U);
+    const int64_t per_simd = (groups * per_group + kComputeUnits * 4 - 1) / (kComputeUnits * 4);
+    if (per_simd >= 8) return "";
+    return " __attribute__((amdgpu_waves_per_eu(1, " + std::to_string(std::max<int64_t>(2, per_simd)) + ")))";
   }
 
   // The same body as a device function over an explicit tile index, for batches. It is
@@ -570,7 +589,7 @@ class Emitter {
     s << "  }\n";
     body_ = &s;
     staged_.clear();
-    if (gq_.empty()) stage_constants(s);
+    if (gq_.empty()) stage_constants(s, prefix_);
     plan_row_groups();
     // phase 1: gathers
     for (size_t i = 0; i < dag_.nodes.size(); ++i)
@@ -823,44 +842,56 @@ class Emitter {
 
   // Constant tables in LDS. A body that reads many single elements of small leaves (the
   // weights and biases of the convolution idiom: 576 + 8 scalars, each broadcast over the
-  // output) would otherwise issue one per-lane global load per scalar in every thread. Each
-  // such leaf's referenced range is copied once per workgroup into csmi_k, and the gathers
-  // read it there (one broadcast LDS read each). The values are the leaf's own, so results
-  // are the same bits. Kernels only (a batch's bodies would add up their tables), and only
-  // with at least kMinStaged such gathers, in at most kMaxStaged floats.
+  // output) would otherwise issue one per-lane global load per scalar in every thread. Those
+  // elements are copied once per workgroup into csmi_k, in the order the gathers read them
+  // (so that consecutive reads are adjacent and load as one wide LDS read), through a table of
+  // (slot, offset) pairs per leaf kept in constant memory; each gather then reads its slot (a
+  // broadcast LDS read). The values are the leaf's own, so results are the same bits. Kernels
+  // only (a batch's bodies would add up their tables), and only with at least kMinStaged such
+  // gathers, in at most kMaxStaged floats.
   static constexpr int64_t kMinStaged = 32, kMaxStaged = 4096;
-  void stage_constants(std::ostringstream &s) {
-    std::map<int, std::pair<int64_t, int64_t>> range;  // leaf -> [lo, hi]
-    std::map<int, int> uses;
+  void stage_constants(std::ostringstream &s, const std::string &prefix) {
+    std::vector<std::pair<int, int64_t>> reads;  // (leaf, offset), in node order
+    std::map<int, std::set<int64_t>> distinct;
     for (size_t i = 0; i < dag_.nodes.size(); ++i) {
       if (dag_.nodes[i].op != CS_OP_EXTRACT) continue;
       int leaf = -1;
       const int64_t off = constant_offset(static_cast<int>(i), &leaf);
       if (off < 0) continue;
-      auto it = range.find(leaf);
-      if (it == range.end())
-        range[leaf] = {off, off};
-      else
-        it->second = {std::min(it->second.first, off), std::max(it->second.second, off)};
-      ++uses[leaf];
+      reads.push_back({leaf, off});
+      distinct[leaf].insert(off);
     }
-    int64_t total = 0, count = 0;
-    std::vector<std::pair<int, std::pair<int64_t, int64_t>>> chosen;
-    for (const auto &kv : range) {
-      const int64_t len = kv.second.second - kv.second.first + 1;
-      if (total + len > kMaxStaged) continue;
-      chosen.push_back({kv.first, {kv.second.first, total}});
-      total += len;
-      count += uses[kv.first];
+    std::set<int> chosen;
+    int64_t total = 0;
+    for (const auto &kv : distinct) {
+      const int64_t n = static_cast<int64_t>(kv.second.size());
+      if (total + n > kMaxStaged) continue;
+      chosen.insert(kv.first);
+      total += n;
     }
-    if (count < kMinStaged) return;
+    if (total < kMinStaged) return;
+    std::map<int, std::vector<std::pair<int64_t, int64_t>>> table;  // leaf -> (slot, offset)
+    for (const auto &rd : reads) {
+      if (!chosen.count(rd.first) || staged_.count(rd)) continue;
+      const int64_t slot = static_cast<int64_t>(staged_.size());
+      staged_[rd] = slot;
+      table[rd.first].push_back({slot, rd.second});
+    }
+    std::ostringstream d;
+    for (const auto &kv : table) {
+      for (int half = 0; half < 2; ++half) {
+        d << "__constant__ unsigned " << prefix << (half ? "_ko" : "_ks") << kv.first << "[" << kv.second.size()
+          << "] = {";
+        for (size_t i = 0; i < kv.second.size(); ++i)
+          d << (i ? (i % 16 ? ", " : ",\n  ") : "") << (half ? kv.second[i].second : kv.second[i].first);
+        d << "};\n";
+      }
+    }
+    decls_ += d.str();
     s << "  __shared__ float csmi_k[" << total << "];\n";
-    for (const auto &c : chosen) {
-      const int64_t len = range[c.first].second - range[c.first].first + 1;
-      s << "  for (unsigned i = threadIdx.x; i < " << len << "u; i += blockDim.x) csmi_k[" << c.second.second
-        << " + i] = p" << c.first << "[" << c.second.first << " + i];\n";
-      staged_[c.first] = c.second;
-    }
+    for (const auto &kv : table)
+      s << "  for (unsigned i = threadIdx.x; i < " << kv.second.size() << "u; i += blockDim.x) csmi_k[" << prefix
+        << "_ks" << kv.first << "[i]] = p" << kv.first << "[" << prefix << "_ko" << kv.first << "[i]];\n";
     s << "  __syncthreads();\n";
   }
 
@@ -888,7 +919,8 @@ class Emitter {
       const cs_node &p = dag_.nodes[ar.param];
       if (p.rank < 2) continue;
       int leaf = -1;
-      if (constant_offset(static_cast<int>(i), &leaf) >= 0 && staged_.count(leaf)) continue;
+      const int64_t off = constant_offset(static_cast<int>(i), &leaf);
+      if (off >= 0 && staged_.count({leaf, off})) continue;
       const std::vector<Row> rows = gather_rows(ar, p);
       const Row &last = rows[static_cast<size_t>(p.rank - 1)];
       if (!last.empty && (last.fractional || !last.unshifted.empty())) continue;
@@ -982,13 +1014,12 @@ class Emitter {
     if (!staged_.empty()) {
       int leaf = -1;
       const int64_t off = constant_offset(id, &leaf);
-      auto it = off < 0 ? staged_.end() : staged_.find(leaf);
+      auto it = off < 0 ? staged_.end() : staged_.find({leaf, off});
       if (it != staged_.end()) {
         uniform_.insert(id);
         s << "  float " << name << "[" << U << "];\n";
         s << "#pragma unroll\n  for (int u = 0; u < " << U << "; ++u) " << name << "[u] = live[u] ? csmi_k["
-          << it->second.second + off - it->second.first << "] : " << float_literal(dag_.nodes[ar.param].value)
-          << ";\n";
+          << it->second << "] : " << float_literal(dag_.nodes[ar.param].value) << ";\n";
         return;
       }
     }
@@ -1187,7 +1218,9 @@ class Emitter {
   std::set<int> uniform_;
   std::map<int, int> fill_of_;
   std::map<std::string, std::string> chunks_;  // aligned source chunks already loaded, by position
-  std::map<int, std::pair<int64_t, int64_t>> staged_;  // leaf -> (first staged offset, its slot in csmi_k)
+  std::map<std::pair<int, int64_t>, int64_t> staged_;  // (leaf, offset) -> its slot in csmi_k
+  std::string decls_;                                   // module-scope declarations the body needs
+  std::string prefix_;                                  // kernel name, for the body's module-scope names
   std::map<std::string, RowGroup> row_groups_;          // rows read together, by leaf and leading indices
   std::map<int, std::string> in_group_;                 // gather node -> its row group
 };
