@@ -788,8 +788,8 @@ def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1, items_per_threa
     """The convolution idiom of the reference's benchmark at its largest setting
     (benchmarks.scala:454-548, 612-631: input [128,32,32,8], 3x3 kernel, 8 filters, one
     layer): a join of 8 per-filter sums of 72 products of translated input channels and
-    broadcast weights, one generated kernel, one element per thread (V=1, 27 us against
-    29 us at V=2 and 56 us at V=4: the kernel holds hundreds of values per element).
+    broadcast weights, one generated kernel, one element per thread (V=1, 9.3 us against
+    25 us at V=2 and 50 us at V=4: only at V=1 are a pixel's channels read together).
     Reported separately (not a headline workload)."""
     from compute import Tensors
     from compute import tensors as tz, trees, workloads
