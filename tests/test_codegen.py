@@ -242,3 +242,53 @@ def test_leaves_of_2_31_elements_are_rejected(offline_tensors):
     ok = trees.ArrayParameter(object(), 0.0, ((1 << 31) - 1,))
     src = T.kernel_source(trees.Binary("+", trees.Extract(ok), trees.Extract(ok)), ((1 << 31) - 1,))
     assert "p0" in src
+
+
+@pytest.mark.parametrize("D,F,V", [(3, 4, 1), (8, 4, 1), (12, 2, 1), (1, 2, 1), (8, 4, 4)])
+def test_constant_table_and_row_groups(D, F, V):
+    """The convolution idiom's body (workloads.convolute): 32 or more constant gathers (weights
+    and biases picked out by split) are copied into an LDS table through a (slot, offset) table
+    per leaf, in the order the gathers read them, and read from it; at one element per vector
+    the channels of one input pixel share one bounds check and load in 16-byte chunks when the
+    channel count is a multiple of 4. Every variant compiles for gfx950."""
+    import re
+
+    from compute import Tensors, workloads
+    T = Tensors(offline=True, vector_width=V, items_per_thread=1, block_size=256)
+    B, H, W = 2, 9, 12
+    x = T.Tensor(np.zeros((B, H, W, D), np.float32))
+    w = T.Tensor(np.zeros((3, 3, D, F), np.float32))
+    b = T.Tensor(np.zeros((F,), np.float32))
+    out = workloads.convolute(T, x, w, b)
+    src = T.kernel_source(trees.Concatenate([p.closure for p in out._parts]), (B, H, W))
+    staged = 9 * D * F + F
+    assert ("__shared__ float csmi_k[%d];" % staged in src) == (staged >= 32)
+    if staged >= 32:
+        # the slots run 0 .. staged-1 in the order the gathers read them
+        slots = sorted(int(k) for k in re.findall(r"\? csmi_k\[(\d+)\]", src))
+        assert slots == list(range(staged))
+        first = [int(k) for k in re.findall(r"\? csmi_k\[(\d+)\]", src)]
+        assert first == sorted(first)
+    groups = re.findall(r"\br(\d+)_ok = live\[u\]", src)
+    # one per kernel offset for the input's channels; without the table, the weights of the
+    # F filters at each (offset, channel) are one row of the weight leaf as well
+    inputs_rows = 9 if D > 1 else 0
+    weight_rows = 9 * D if staged < 32 and F > 1 else 0
+    assert len(groups) == (inputs_rows + weight_rows if V == 1 else 0)
+    chunks = re.findall(r"\br\d+_c(\d+) = \*\(const csmi_f4\*\)", src)
+    assert len(chunks) == (9 * D // 4 if V == 1 and D % 4 == 0 else 0)
+    compiles(src)
+
+
+def test_register_budget_of_a_small_grid():
+    """A body of more than 256 nodes whose grid gives at most 2 waves per SIMD of 256 CUs is
+    compiled with amdgpu_waves_per_eu(1, 2); the same body on a grid 8 times larger is not."""
+    from compute import Tensors, workloads
+    T = Tensors(offline=True, vector_width=1, items_per_thread=1, block_size=256)
+    for batch, hinted in ((128, True), (1024, False)):
+        x = T.Tensor(np.zeros((batch, 32, 32, 8), np.float32))
+        w = T.Tensor(np.zeros((3, 3, 8, 8), np.float32))
+        b = T.Tensor(np.zeros((8,), np.float32))
+        out = workloads.convolute(T, x, w, b)
+        src = T.kernel_source(trees.Concatenate([p.closure for p in out._parts]), (batch, 32, 32))
+        assert ("__attribute__((amdgpu_waves_per_eu(1, 2)))" in src) == hinted

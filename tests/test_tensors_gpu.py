@@ -1,5 +1,7 @@
 """The reference's known answers through the gfx950 back-end, and agreement with the
 oracle (the reference's kernels run as host C) on the same graphs."""
+import re
+
 import numpy as np
 import pytest
 
@@ -101,26 +103,45 @@ def _params(closure):
     return trees.parameters(closure)
 
 
-def test_convolution_idiom_bit_exact(gpu_tensors):
+CONV_CASES = [
+    # (depth, filters, vector width): 4 wide; constant table + a row of 3 read per element;
+    # table + rows of 8 and 12 in 16-byte chunks; no table (18 + 2 constants), the 2 filters'
+    # weights read as rows
+    (3, 4, 4), (3, 4, 1), (8, 4, 1), (12, 2, 1), (1, 2, 1),
+]
+
+
+@pytest.mark.parametrize("D,F,V", CONV_CASES)
+def test_convolution_idiom_bit_exact(D, F, V):
     """The reference's convolution idiom (TensorsSpec.scala:144-210, benchmarks.scala:454-548:
     a join of per-filter sums of translated input channels times broadcast weights) on random
     data with padding at every edge: one kernel, bit-exact with the FP32 program (the sums
     keep the fold's order, no contraction). The same translated view taken once per term is
-    gathered once."""
-    from compute import trees
+    gathered once. The weights and biases, when there are 32 or more, are read from a table in
+    LDS, and at one element per vector a pixel's channels are read as one row: neither changes
+    a bit."""
+    from compute import Tensors, trees
     from oracle import inputs
-    T = gpu_tensors
-    B, H, W, D, F = 2, 9, 11, 3, 4
-    x = inputs.normal(B * H * W * D, 61).reshape(B, H, W, D)
-    w = inputs.normal(3 * 3 * D * F, 62).reshape(3, 3, D, F)
-    b = inputs.normal(F, 63)
-    tx, tw, tb = T.Tensor(x), T.Tensor(w), T.Tensor(b)
-    out = spec_cases.convolute(T, tx, tw, tb)
-    clo = trees.Concatenate([p.closure for p in out._parts])
-    got = np.asarray(out.flatArray(), np.float32)
-    data = {id(tx): x, id(tw): w, id(tb): b}
-    leaves = [data[id(p.id)] for p in fp64._param_order(clo)]  # the kernel's first-visit order
-    want = fp64.evaluate(clo, (B, H, W), leaves, "fp32").ravel()
-    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
-    src = T.kernel_source(clo, (B, H, W))
-    assert "nodes, 3 leaves" in src
+    T = Tensors(0, vector_width=V)
+    try:
+        B, H, W = 2, 9, 12 if V == 4 else 11
+        x = inputs.normal(B * H * W * D, 61).reshape(B, H, W, D)
+        w = inputs.normal(3 * 3 * D * F, 62).reshape(3, 3, D, F)
+        b = inputs.normal(F, 63)
+        tx, tw, tb = T.Tensor(x), T.Tensor(w), T.Tensor(b)
+        out = spec_cases.convolute(T, tx, tw, tb)
+        clo = trees.Concatenate([p.closure for p in out._parts])
+        got = np.asarray(out.flatArray(), np.float32)
+        data = {id(tx): x, id(tw): w, id(tb): b}
+        leaves = [data[id(p.id)] for p in fp64._param_order(clo)]  # the kernel's first-visit order
+        want = fp64.evaluate(clo, (B, H, W), leaves, "fp32").ravel()
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+        src = T.kernel_source(clo, (B, H, W))
+        assert "nodes, 3 leaves" in src
+        staged = 9 * D * F + F >= 32
+        assert ("csmi_k[" in src) == staged
+        # rows read together: a pixel's channels, and without the table the filters' weights
+        assert bool(re.search(r"\br\d+_ok = live\[u\]", src)) == (V == 1 and (D > 1 or (not staged and F > 1)))
+        assert bool(re.search(r"\br\d+_c0 = \*\(const csmi_f4\*\)", src)) == (V == 1 and D % 4 == 0)
+    finally:
+        T.close()
