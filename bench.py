@@ -161,21 +161,28 @@ class StreamSpread(object):
         rem = count % self.block
         for st in self.streams:
             check(lib.cs_stream_synchronize(st))
-        starts, ends = [], []
-        for st in self.streams:
-            e = ctypes.c_void_p()
-            check(lib.cs_event_record(st, ctypes.byref(e)))
-            starts.append(e)
+        # each stream's start event goes in just before its first graph, so the window opens
+        # when the first stream's work is submitted, not before the other streams' events
+        starts, ends, used = [], [], []
+
+        def launch(j, st, g):
+            if j not in used:
+                e = ctypes.c_void_p()
+                check(lib.cs_event_record(st, ctypes.byref(e)))
+                starts.append(e)
+                used.append(j)
+            check(lib.cs_graph_launch(st, g))
+
         for _ in range(count // self.block):
-            for st, g in zip(self.streams, self.full):
-                check(lib.cs_graph_launch(st, g))
+            for j, (st, g) in enumerate(zip(self.streams, self.full)):
+                launch(j, st, g)
         if rem:
-            for st, g in zip(self.streams, self.tails[rem]):
+            for j, (st, g) in enumerate(zip(self.streams, self.tails[rem])):
                 if g is not None:
-                    check(lib.cs_graph_launch(st, g))
-        for st in self.streams:
+                    launch(j, st, g)
+        for j in used:
             e = ctypes.c_void_p()
-            check(lib.cs_event_record(st, ctypes.byref(e)))
+            check(lib.cs_event_record(self.streams[j], ctypes.byref(e)))
             ends.append(e)
         for st in self.streams:
             check(lib.cs_stream_synchronize(st))

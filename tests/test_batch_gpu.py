@@ -110,3 +110,41 @@ def test_c5_full_size(gpu_tensors):
             assert fp64.within_bound(each[i], value, bound).all(), i
     finally:
         eset.release()
+
+
+def test_row_groups_in_a_batch(gpu_tensors):
+    """Bodies at one element per vector whose gathers read a pixel's channels (split along the
+    last dimension, some translated to the edges) share one row per pixel in the batch's
+    device functions as in stand-alone kernels: the batch is bit-identical to per-expression
+    launches and within the oracle's bound."""
+    from compute import _capi as capi
+    from oracle import inputs
+    T = gpu_tensors
+    exprs = []
+    for k, (h, w, d) in enumerate([(5, 9, 8), (17, 3, 12), (4, 33, 6), (64, 70, 4)]):
+        x = inputs.uniform(h * w * d, 300 + k).reshape(h, w, d)
+        tx = T.Tensor(x)
+        c = list(tx.split(2))
+        t = c[0] * c[1] + c[d - 1] - c[2].translate([1, -1]) * c[d // 2]
+        exprs.append(workloads.Expr(k, t, [tx], [x], [], ["direct"]))
+    eset = bm.ExpressionSet(T, exprs)
+    try:
+        cfg = {"vector_width": 1, "items_per_thread": 2, "block_size": 256, "nontemporal_store": 0}
+        fused = bm.compile_many(T, exprs, [cfg] * len(exprs))
+        for i, f in enumerate(fused):
+            eset.launch(i, f, 256)
+        each = eset.outputs()
+        eset.clear_outputs()
+        b = bm.Batch(eset, fused, block=256)
+        b.launch()
+        T.synchronize()
+        got = eset.outputs()
+        for i, e in enumerate(exprs):
+            assert np.array_equal(got[i].view(np.uint32), each[i].view(np.uint32)), i
+            value, bound = fp64.evaluate_with_bound(e.tensor.closure, e.shape, e.leaf_data, ulps=2.0)
+            assert fp64.within_bound(got[i], value, bound).all(), i
+        b.release()
+        for f in fused:
+            capi.lib.cs_fused_release(f)
+    finally:
+        eset.release()

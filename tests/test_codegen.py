@@ -292,3 +292,31 @@ def test_register_budget_of_a_small_grid():
         out = workloads.convolute(T, x, w, b)
         src = T.kernel_source(trees.Concatenate([p.closure for p in out._parts]), (batch, 32, 32))
         assert ("__attribute__((amdgpu_waves_per_eu(1, 2)))" in src) == hinted
+
+
+def test_row_groups_in_batch_bodies(offline_tensors):
+    """Rows read together also in a persistent batch's device functions (global-address-space
+    row pointers); the batch source compiles for gfx950."""
+    import re
+
+    from compute import batch as bm
+    T = offline_tensors
+
+    class Expr(object):
+        pass
+
+    exprs = []
+    for k in range(3):
+        c = list(T.Tensor(np.zeros((6, 7 + k, 8), np.float32)).split(2))
+        e = Expr()
+        e.tensor = c[0] * c[1] + c[2] - c[5] * c[7] + c[3].translate([1, 0])
+        e.shape = tuple(e.tensor.shape)
+        exprs.append(e)
+    descs, keep = bm._descs(exprs, [{"vector_width": 1, "items_per_thread": 2, "block_size": 256}] * 3,
+                            capi.MATH_PRECISE)
+    src = ctypes.c_void_p()
+    capi.check(capi.lib.cs_batch_generate_source(descs, 3, 256, ctypes.byref(src), None))
+    text = capi.take_string(src.value)
+    assert len(re.findall(r"\br\d+_ok = live\[u\]", text)) == 3  # c0 c1 c2 c5 c7 of each body
+    assert "const CSMI_G float* r" in text and "csmi_k" not in text
+    compiles(text)
