@@ -921,7 +921,8 @@ def profile_run(T, capi, what, count=512):
     """Launches of one workload's kernels without graphs or extra streams, for a profiler to
     time per dispatch: `c2` (the headline kernel over the headline's rotating sets, plain
     launches one after another on the launch stream), `c3` (the row reduction), `c4` (one
-    call of each GEMM variant, several times)."""
+    call of each GEMM variant, several times), `c5` (the batch, then each expression once),
+    `conv` (the convolution idiom's kernel at the benchmark's launch shape)."""
     from compute import autotune
     from compute import tensors as tz
     lib = capi.lib
@@ -983,8 +984,34 @@ def profile_run(T, capi, what, count=512):
         for f in fs:
             lib.cs_fused_release(f)
         eset.release()
+    elif what == "conv":
+        from compute import trees, workloads
+        Bn, H, W, D, F = 128, 32, 32, 8, 8
+        n_in, n_w, n_out = Bn * H * W * D, 3 * 3 * D * F, Bn * H * W * F
+        x, o = T._alloc(n_in * 4), T._alloc(n_out * 4)
+        wb, bb = T._alloc(n_w * 4), T._alloc(F * 4)
+        capi.check(lib.cs_fill_normal(T.stream, x.ptr, n_in, 6001))
+        capi.check(lib.cs_fill_normal(T.stream, wb.ptr, n_w, 6101))
+        capi.check(lib.cs_fill_normal(T.stream, bb.ptr, F, 6102))
+        X = tz.CachedTensor(T, (Bn, H, W, D), 0.0, x.retain())
+        Wt = tz.CachedTensor(T, (3, 3, D, F), 0.0, wb.retain())
+        Bt = tz.CachedTensor(T, (F,), 0.0, bb.retain())
+        out = workloads.convolute(T, X, Wt, Bt)
+        # the benchmark's launch shape: one element per thread, 256 threads
+        f, leaves = autotune.compile_with(T, trees.Concatenate([q.closure for q in out._parts]), (Bn, H, W),
+                                          {"vector_width": 1, "items_per_thread": 1, "block_size": 256,
+                                           "nontemporal_store": 0})
+        by_id = {id(X): x, id(Wt): wb, id(Bt): bb}
+        ptrs = (ctypes.c_void_p * len(leaves))(*[by_id[id(l.id)].ptr for l in leaves])
+        for _ in range(count // 4):
+            capi.check(lib.cs_fused_launch(T.stream, f, ptrs, len(leaves), o.ptr))
+        T.synchronize()
+        log("convolution idiom: %d plain launches" % (count // 4))
+        lib.cs_fused_release(f)
+        for h in (x, o, wb, bb, X, Wt, Bt):
+            h.release()
     else:
-        raise SystemExit("--profile takes c2, c3, c4 or c5")
+        raise SystemExit("--profile takes c2, c3, c4, c5 or conv")
 
 
 def cpu_model():
@@ -1098,7 +1125,7 @@ def main():
                     help="first copy a buffer of this many MiB device-to-device 8 times (a known-size "
                          "reference for hardware byte counters)")
     ap.add_argument("--profile", default=None,
-                    help="c2, c3, c4 or c5: only that workload's kernels, launched without graphs (for rocprofv3)")
+                    help="c2, c3, c4, c5 or conv: only that workload's kernels, launched without graphs (for rocprofv3)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every workload here runs on a single MI355X, "
