@@ -294,6 +294,10 @@ struct Options {
   bool sum = false;  // the kernel adds its single output into FP64 block partials instead of storing it
 };
 
+// A body of more than this many canonical nodes is large: it is compiled for its own workgroup
+// size (launch_bound) and launched by default at one element per vector and per thread.
+constexpr size_t kLargeBody = 256;
+
 Options resolve_options(const cs_fused_options *o, const Dag &dag) {
   Options r;
   if (o) {
@@ -310,10 +314,14 @@ Options resolve_options(const cs_fused_options *o, const Dag &dag) {
   int64_t last = dag.out_shape.empty() ? 1 : dag.out_shape.back();
   int64_t total = 1;
   for (auto s : dag.out_shape) total *= s;
-  if (r.V == 0) r.V = 4;
+  // Defaults: 4 elements per vector, 2 vectors per thread from 2^22 elements. A large body
+  // holds hundreds of values per element, so it defaults to 1 and 1 (the convolution idiom:
+  // 9.3 us at V=1 against 25 us at V=2 and 50 us at V=4; two vectors per thread spill, 129 us).
+  const bool large = dag.nodes.size() > kLargeBody;
+  if (r.V == 0) r.V = large ? 1 : 4;
   if (r.V != 1 && r.V != 2 && r.V != 4 && r.V != 8) fail(CS_INVALID_VALUE, "vector_width must be 1, 2, 4 or 8");
   while (r.V > 1 && last % r.V != 0) r.V /= 2;
-  if (r.U == 0) r.U = total >= (int64_t(1) << 22) ? 2 : 1;
+  if (r.U == 0) r.U = !large && total >= (int64_t(1) << 22) ? 2 : 1;
   if (r.U < 1 || r.U > 8) fail(CS_INVALID_VALUE, "items_per_thread must be 1..8");
   if (r.block == 0) r.block = 256;
   if (r.block < 64 || r.block > 1024 || r.block % 64 != 0)
@@ -369,7 +377,6 @@ std::string canonical_key(const Dag &dag, const Options &o) {
 // that caps a thread at 128 VGPRs, which a large body (a convolution's hundreds of terms)
 // exceeds and spills. Those are compiled for their own workgroup size only, which gives them
 // up to 512 registers.
-constexpr size_t kLargeBody = 256;  // canonical nodes
 int launch_bound(const Dag &dag, const Options &o) { return dag.nodes.size() > kLargeBody ? o.block : 1024; }
 
 // ------------------------------------------------------------------------ code generation

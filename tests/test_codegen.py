@@ -320,3 +320,21 @@ def test_row_groups_in_batch_bodies(offline_tensors):
     assert len(re.findall(r"\br\d+_ok = live\[u\]", text)) == 3  # c0 c1 c2 c5 c7 of each body
     assert "const CSMI_G float* r" in text and "csmi_k" not in text
     compiles(text)
+
+
+def test_large_bodies_default_to_one_element_per_thread(offline_tensors):
+    """Without an explicit launch shape a body of more than 256 canonical nodes (the
+    convolution idiom) runs one element per vector and per thread; a small body keeps 4
+    elements per vector and 2 vectors per thread from 2^22 elements."""
+    from compute import workloads
+    T = offline_tensors
+    x = T.Tensor(np.zeros((128, 32, 32, 8), np.float32))
+    w = T.Tensor(np.zeros((3, 3, 8, 8), np.float32))
+    b = T.Tensor(np.zeros((8,), np.float32))
+    out = workloads.convolute(T, x, w, b)
+    clo = trees.Concatenate([p.closure for p in out._parts])
+    src = T.kernel_source(clo, (128, 32, 32))
+    assert ", V=1 U=1\n" in src.split("\n", 1)[0] + "\n"
+    y = T.Tensor(np.zeros((1024, 4096), np.float32))
+    src = T.kernel_source((y * y).closure, (1024, 4096))
+    assert ", V=4 U=2\n" in src.split("\n", 1)[0] + "\n"
