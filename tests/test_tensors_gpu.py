@@ -108,6 +108,7 @@ CONV_CASES = [
     # table + rows of 8 and 12 in 16-byte chunks; no table (18 + 2 constants), the 2 filters'
     # weights read as rows
     (3, 4, 4), (3, 4, 1), (8, 4, 1), (12, 2, 1), (1, 2, 1),
+    (8, 4, 0),  # no launch shape given: a body this large runs one element per vector
 ]
 
 
@@ -124,7 +125,7 @@ def test_convolution_idiom_bit_exact(D, F, V):
     from oracle import inputs
     T = Tensors(0, vector_width=V)
     try:
-        B, H, W = 2, 9, 12 if V == 4 else 11
+        B, H, W = 2, 9, 12 if V in (0, 4) else 11
         x = inputs.normal(B * H * W * D, 61).reshape(B, H, W, D)
         w = inputs.normal(3 * 3 * D * F, 62).reshape(3, 3, D, F)
         b = inputs.normal(F, 63)
@@ -138,6 +139,9 @@ def test_convolution_idiom_bit_exact(D, F, V):
         assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
         src = T.kernel_source(clo, (B, H, W))
         assert "nodes, 3 leaves" in src
+        if V == 0:
+            assert ", V=1 U=1" in src.split("\n", 1)[0]
+            V = 1
         staged = 9 * D * F + F >= 32
         assert ("csmi_k[" in src) == staged
         # rows read together: a pixel's channels, and without the table the filters' weights
