@@ -421,20 +421,21 @@ class Emitter {
     return " __attribute__((amdgpu_waves_per_eu(1, " + std::to_string(std::max<int64_t>(2, per_simd)) + ")))";
   }
 
-  // The same body as a device function over an explicit tile index, for batches. It is
-  // not inlined: inlined into one switch, hundreds of bodies share values hoisted across
-  // the cases and the kernel's register count (and occupancy) is set by the sum, not the
-  // largest. Pointers carry the global address space (a function parameter is otherwise
-  // generic and every access a flat one) and the workgroup size is the batch's constant.
+  // The same body as a device function over a run of tiles [tile_begin, tile_end), for
+  // batches: one call per work item, the loop inside. It is not inlined: inlined into one
+  // switch, hundreds of bodies share values hoisted across the cases and the kernel's
+  // register count (and occupancy) is set by the sum, not the largest. Pointers carry the
+  // global address space (a function parameter is otherwise generic and every access a flat
+  // one) and the workgroup size is the batch's constant.
   std::string device_function(const std::string &name, int block, int64_t *vectors_out) {
     gq_ = "CSMI_G ";
     bdim_ = std::to_string(block) + "u";
     std::ostringstream s;
-    s << "__device__ __noinline__ void " << name << "(const unsigned tile, ";
+    s << "__device__ __noinline__ void " << name << "(const unsigned tile_begin, const unsigned tile_end, ";
     params(s);
-    s << ") {\n";
+    s << ") {\n  for (unsigned tile = tile_begin; tile < tile_end; ++tile) {\n";
     body(s, "tile", vectors_out);
-    s << "}\n";
+    s << "  }\n}\n";
     gq_.clear();
     bdim_ = "blockDim.x";
     return s.str();
@@ -1355,15 +1356,14 @@ BatchPlan plan_batch(const std::vector<BatchExpr> &members, int block, int math_
   src << "  while (w < nitems) {\n";
   src << "    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);\n";
   src << "    const uint4 it = items[w];\n    const float* const* a = args + it.x * " << plan.max_args << "u;\n";
-  src << "    for (unsigned t = it.y; t < it.y + it.z; ++t) {\n";
-  src << "      switch (it.x) {\n";
+  src << "    switch (it.x) {\n";
   for (int i = 0; i < n; ++i) {
     const int nl = members[i].dag->nleaves;
-    src << "        case " << i << ": expr_" << i << "(t";
+    src << "      case " << i << ": expr_" << i << "(it.y, it.y + it.z";
     for (int k = 0; k < nl; ++k) src << ", (const CSMI_G float*)a[" << k << "]";
     src << ", (CSMI_G float*)a[" << nl << "]); break;\n";
   }
-  src << "        default: break;\n      }\n    }\n";
+  src << "      default: break;\n    }\n";
   src << "    __syncthreads();\n    w = __builtin_amdgcn_readfirstlane(next);\n    __syncthreads();\n  }\n}\n";
   plan.source = src.str();
   std::vector<int> order(static_cast<size_t>(n));

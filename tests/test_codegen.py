@@ -183,8 +183,9 @@ def test_structural_key_ignores_leaf_identity(offline_tensors):
 
 def test_batch_source_offline(offline_tensors):
     """The persistent batch's source (cs_batch_generate_source): one non-inlined device
-    function per expression over global-address-space pointers, a constant workgroup size,
-    and one switch case per expression; the text compiles for gfx950."""
+    function per expression over global-address-space pointers and a run of tiles, a
+    constant workgroup size, and one switch case per expression; the text compiles for
+    gfx950."""
     from compute import batch as bm
     from compute import workloads
     T = offline_tensors
@@ -198,7 +199,9 @@ def test_batch_source_offline(offline_tensors):
     assert "#define CSMI_G __attribute__((address_space(1)))" in text
     assert "(tile) * (512u * 2u)" in text and "blockDim" not in text.split("batch_kernel")[0]
     for i, e in enumerate(exprs):
-        assert "case %d: expr_%d(t" % (i, i) in text
+        assert "case %d: expr_%d(it.y, it.y + it.z" % (i, i) in text
+    # one call per work item: the run of tiles is looped inside the body's function
+    assert text.count("for (unsigned tile = tile_begin; tile < tile_end; ++tile)") == len(exprs)
     # every tile of every expression is in exactly one work item
     tiles = sum(-(-(e.out_elements // bm.effective_vector_width(e.shape, 2)) // (512 * 2)) for e in exprs)
     assert 1 <= items.value <= tiles
