@@ -10,8 +10,9 @@ sets rotate (512 MiB in total, twice the 256 MB Infinity Cache) so every launch 
 from HBM. The K timed launches are spread over 3 streams (launch i on stream
 (i mod 16) mod 3, as the reference's CommandQueuePool leases a queue per command and its JMH
 benchmarks run independent iterations from every core at once, benchmarks.scala:31-32, 56),
-each stream replaying captured hipGraphs of its share, and timed from the earliest start
-event to the latest end event of the streams. Beside it: the same K launches on one stream, and a window
+each stream replaying captured hipGraphs of its share, and timed from an event recorded on
+the first stream before any launch is submitted to the latest end event of the streams.
+Beside it: the same K launches on one stream, and a window
 of at least 4096 launches over 1 to 4 streams.
 
 The same line carries the other workloads under "also": a*b+c on [1024,1024] and its sum,
@@ -113,9 +114,9 @@ class StreamSpread(object):
     evenly). This is the reference's CommandQueuePool, which leases one of its queues to each
     command (OpenCL.scala:423-448, 1313-1322; five queues per device in gpu.scala:26). Each
     stream replays captured graphs of its own launches, so one launch's ramp-down overlaps
-    another stream's work. A run is timed from the earliest start event to the latest end
-    event of the streams, which are idle before it and synchronised after it. With k = 1 it is
-    the launch stream alone."""
+    another stream's work. A run is timed from an event on the first stream, recorded before
+    any launch is submitted, to the latest end event of the streams, which are idle before it
+    and synchronised after it. With k = 1 it is the launch stream alone."""
 
     ROUNDS = 16  # rounds of the sets in one replay of the full graphs
 
@@ -161,15 +162,18 @@ class StreamSpread(object):
         rem = count % self.block
         for st in self.streams:
             check(lib.cs_stream_synchronize(st))
-        # each stream's start event goes in just before its first graph, so the window opens
-        # when the first stream's work is submitted, not before the other streams' events
+        # The window opens with one event on the first stream, recorded before any graph is
+        # submitted: work on the other streams is submitted after it, so it cannot start
+        # earlier, and no further event sits between the submissions. It closes with the last
+        # of the streams' end events.
         starts, ends, used = [], [], []
 
         def launch(j, st, g):
-            if j not in used:
+            if not used:
                 e = ctypes.c_void_p()
                 check(lib.cs_event_record(st, ctypes.byref(e)))
                 starts.append(e)
+            if j not in used:
                 used.append(j)
             check(lib.cs_graph_launch(st, g))
 
@@ -186,12 +190,10 @@ class StreamSpread(object):
             ends.append(e)
         for st in self.streams:
             check(lib.cs_stream_synchronize(st))
-        first, last = None, None
-        for e0, e1 in zip(starts, ends):
-            a, b = ctypes.c_float(), ctypes.c_float()
-            check(lib.cs_event_elapsed_ms(starts[0], e0, ctypes.byref(a)))
+        first, last = 0.0, None
+        for e1 in ends:
+            b = ctypes.c_float()
             check(lib.cs_event_elapsed_ms(starts[0], e1, ctypes.byref(b)))
-            first = a.value if first is None else min(first, a.value)
             last = b.value if last is None else max(last, b.value)
         for e in starts + ends:
             lib.cs_event_release(e)
