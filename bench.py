@@ -793,7 +793,7 @@ def bench_c5(T, capi, sweep=False, out_dir=None, n=512, seed=5001, nontemporal=1
     return res
 
 
-def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1, items_per_thread=0, block_size=256):
+def bench_conv(T_outer, capi, steps=256, sets=4, vector_width=1, items_per_thread=1, block_size=256):
     """The convolution idiom of the reference's benchmark at its largest setting
     (benchmarks.scala:454-548, 612-631: input [128,32,32,8], 3x3 kernel, 8 filters, one
     layer): a join of 8 per-filter sums of 72 products of translated input channels and
