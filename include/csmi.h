@@ -229,6 +229,12 @@ typedef struct cs_fused_options {
   int32_t launch_order;       /* cs_launch_order: which outputs a workgroup computes together */
 } cs_fused_options;
 
+/* The automatic launch shape: vector_width 4 (halved until it divides the innermost extent),
+ * items_per_thread 2 from 2^22 output positions and 1 below, 256 threads. An expression of
+ * more than 256 canonical nodes (such as the convolution idiom) holds hundreds of values per
+ * element: it gets vector_width 1 and items_per_thread 1, and is compiled for its own
+ * workgroup size (a larger launch of it is CS_INVALID_WORK_GROUP_SIZE). */
+
 /* CS_ORDER_AUTO: when a leaf is read through a view whose source rows run along an outer
  * output dimension (a transpose), workgroups take 2-D tiles across that dimension, so the
  * leaf's rows are read in whole cache lines; otherwise, and for fused sums, the linear order.
