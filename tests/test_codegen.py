@@ -284,17 +284,21 @@ def test_constant_table_and_row_groups(D, F, V):
 
 
 def test_register_budget_of_a_small_grid():
-    """A body of more than 256 nodes whose grid gives at most 2 waves per SIMD of 256 CUs is
-    compiled with amdgpu_waves_per_eu(1, 2); the same body on a grid 8 times larger is not."""
+    """A body of more than 256 nodes whose grid gives few waves per SIMD of 256 CUs (4 SIMDs
+    each) is compiled with the budget of that many waves, at least 2: 2048 waves and fewer get
+    amdgpu_waves_per_eu(1, 2), 3072 waves (1, 3); a grid of 8 or more waves per SIMD gets none."""
     from compute import Tensors, workloads
     T = Tensors(offline=True, vector_width=1, items_per_thread=1, block_size=256)
-    for batch, hinted in ((128, True), (1024, False)):
+    for batch, hint in ((32, 2), (128, 2), (192, 3), (1024, None)):
         x = T.Tensor(np.zeros((batch, 32, 32, 8), np.float32))
         w = T.Tensor(np.zeros((3, 3, 8, 8), np.float32))
         b = T.Tensor(np.zeros((8,), np.float32))
         out = workloads.convolute(T, x, w, b)
         src = T.kernel_source(trees.Concatenate([p.closure for p in out._parts]), (batch, 32, 32))
-        assert ("__attribute__((amdgpu_waves_per_eu(1, 2)))" in src) == hinted
+        if hint is None:
+            assert "amdgpu_waves_per_eu" not in src
+        else:
+            assert "__attribute__((amdgpu_waves_per_eu(1, %d)))" % hint in src
 
 
 def test_row_groups_in_batch_bodies(offline_tensors):
