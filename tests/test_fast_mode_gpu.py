@@ -197,3 +197,28 @@ def test_c2_fast_full_size(fast):
     dh = fp64.ulp_distance(o, host)
     print("c2 fast: y bit-exact %s; output max %d ulp from the correctly rounded program (mean %.3f), "
           "max %d ulp from refcl-host fast" % (bool(np.array_equal(y_gpu, yv.ravel())), d.max(), d.mean(), dh.max()))
+
+
+@pytest.mark.parametrize("D,V", [(8, 1), (3, 4)])
+def test_convolution_idiom_fast(D, V):
+    """The convolution idiom in the fast mode, with its weights read from the LDS table and (at
+    one element per vector) a pixel's channels read as one row: every output within the fast
+    bound of the FP64 oracle (contracted and reassociated sums of 72 products)."""
+    from compute import Tensors, trees
+    import spec_cases
+    B, H, W, F = 2, 9, 12, 4
+    x = inputs.normal(B * H * W * D, 71).reshape(B, H, W, D)
+    w = inputs.normal(9 * D * F, 72).reshape(3, 3, D, F)
+    b = inputs.normal(F, 73)
+    with Tensors(0, math_mode=capi.MATH_FAST, vector_width=V) as T:
+        tx, tw, tb = T.Tensor(x), T.Tensor(w), T.Tensor(b)
+        out = spec_cases.convolute(T, tx, tw, tb)
+        clo = trees.Concatenate([p.closure for p in out._parts])
+        got = np.asarray(out.flatArray(), np.float32).reshape(B, H, W, F)
+        assert "csmi_k[" in T.kernel_source(clo, (B, H, W))
+        data = {id(tx): x, id(tw): w, id(tb): b}
+    for f, part in enumerate(out._parts):
+        value, bound = fp64.evaluate_with_bound(part.closure, (B, H, W),
+                                                [data[id(p.id)] for p in fp64._param_order(part.closure)],
+                                                mode="fast")
+        assert fp64.within_bound(got[..., f], value, bound).all(), f
